@@ -1,0 +1,143 @@
+/*
+ * paragram_hip.h -- C ABI of libparagram_hip.so, the gfx950 (MI355X) engine
+ * behind paragram_amd.voronoi_diagram / paragram_amd.power_diagram.
+ *
+ * The reference exposes its engine as the pybind module paragram._backend._C
+ * with two functions (paragram/csrc/ext.cpp:10-29, declared at
+ * paragram/csrc/laguerre/laguerre.h:12-28):
+ *     build_voronoi_ultra(points, knn, debug, initial_guesses,
+ *                         initial_guesses_offsets, leaf_size)
+ *     build_laguerre_voronoi_ultra(points, weights, ...)
+ * each returning (adjacency int32[E], offsets int32[N+1], status int32[N]).
+ * Those two calls are split here into the stages their C++ bodies run
+ * (voronoi_ultra.cu:143-376, laguerre_ultra.cu:176-454), so that the host
+ * (Python over ctypes) owns every allocation and the single device-to-host
+ * read of E.
+ *
+ * Conventions
+ *   - Every pointer argument except `stream` and host out-parameters is
+ *     device memory of the current HIP device; sizes are element counts.
+ *   - `stream` is a hipStream_t (0 = the null stream); all work is ordered on
+ *     it and no call synchronises the host, except where stated.
+ *   - The library allocates nothing: workspaces are sized by the *_bytes
+ *     calls and passed in.
+ *   - Return value 0 = success; otherwise pg_last_error() (thread-local)
+ *     describes the failure.  Code 1 = invalid argument, 2 = HIP error.
+ *   - N <= 67,108,863 (2^26 - 1) points per call.
+ */
+#ifndef PARAGRAM_HIP_H
+#define PARAGRAM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_ABI_VERSION 1
+
+/* Number of statistics counters written by the calls that take `stats`
+ * (uint64 each): 0 FP64 rechecks, 1 unresolved, 2 inconsistent,
+ * 3 capacity overflows, 4 clips, 5 candidates tested, 6 nodes visited,
+ * 7 cells, 8 asymmetric edges, 9 entries to empty cells, 10 leaves visited,
+ * 11 candidates rejected by the support test, 15 edge total E. */
+#define PG_NUM_STATS 16
+
+/* ABI version compiled into the library (== PG_ABI_VERSION). */
+int pg_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *pg_last_error(void);
+
+/* Row stride of the capacity-0 tier (rows of the rare larger cells live after it). */
+int pg_row_capacity(void);
+
+/* Bytes of the spatial-index workspace for n points.
+ * Replaces the allocations of pwr_bvh.cu:228-270 (cudaMalloc of boxes,
+ * nodes, primIDs, parents, counters) and the points4 tensor of
+ * voronoi_ultra.cu:186. */
+int pg_index_workspace_bytes(int64_t n, uint64_t *bytes);
+
+/* Builds the LBVH over points (n x 3 float32, optional weights n float32):
+ * exact AABB and FP32 clipping box, Morton sort, Karras hierarchy, AABB and
+ * max-weight refit.  leaf_size < 0 selects the default.
+ * Replaces pack_points_kernel / combine_points_weights_kernel
+ * (voronoi_ultra.cu:30-37, laguerre_ultra.cu:31-41), build_cubql_bvh_and_knn
+ * and build_pwr_bvh_and_knn (pwr_bvh.cu:217-339) and the root-box read of
+ * voronoi_ultra.cu:234-238 (the box stays on the device). */
+int pg_build_index(int64_t n, const float *points, const float *weights, int32_t leaf_size, void *workspace,
+                   uint64_t workspace_bytes, void *stream);
+
+/* Persistent waves used by pg_compute_cells on the current device. */
+int pg_default_waves(int32_t *waves);
+
+/* Bytes of cell scratch for `waves` persistent waves and launches over at
+ * most max_cells cells (per-wave state of both capacity tiers plus the
+ * tier-1 queue).
+ * Replaces ConvexCellMemory::required_bytes (voronoi_convex_cell.cuh:37-44). */
+int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes);
+
+/* int32 elements of the row buffer of a launch over `cells` cells. */
+int pg_row_buffer_elems(int64_t cells, uint64_t *elems);
+
+/* Builds the cells of the Morton-sorted range [begin, end): each cell's
+ * sorted neighbour row (original ids) goes to rows + row_offsets[i] (i the
+ * sorted id), its row length to counts[orig] and its status to status[orig].
+ * rows holds pg_row_buffer_elems(end - begin) elements.
+ * guesses / guess_offsets: optional CSR of candidate ids clipped first
+ * (original ids, or sorted ids when guesses_sorted != 0); negative ids skip.
+ * stats: PG_NUM_STATS counters, accumulated.
+ * Replaces the chunk loop and compute_bvh_{voronoi,power_diagram}_kernel
+ * (voronoi_ultra.cu:39-141,283-305; laguerre_ultra.cu:43-174,337-357). */
+int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t *rows, int64_t *row_offsets,
+                     int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                     int32_t waves, void *stream);
+
+/* Bytes of temporary storage for pg_scan_counts over n values. */
+int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes);
+
+/* offsets[0..n] = exclusive prefix sum of counts[0..n); offsets[n] = total.
+ * Replaces torch::cat + torch::cumsum of voronoi_ultra.cu:347-350. */
+int pg_scan_counts(int64_t n, const int32_t *counts, int32_t *offsets, void *tmp, void *stream);
+
+/* out[i - begin] = counts[perm[i]] for sorted i in [begin, end). */
+int pg_chunk_counts(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
+                    int32_t *out, void *stream);
+
+/* Copies the rows of sorted cells [begin, end) from rows + row_offsets[i]
+ * to packed + local_offsets[i - begin]. */
+int pg_pack_rows(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
+                 const int32_t *local_offsets, const int32_t *rows, const int64_t *row_offsets, int32_t *packed,
+                 void *stream);
+
+/* Counts asymmetric edges and entries pointing at empty cells (stats[8],
+ * stats[9]) and copies offsets[n] to stats[15].  Source rows are addressed
+ * by sorted cell: src + src_offsets[i], or src + i * src_stride when
+ * src_offsets is null. */
+int pg_check_symmetry(int64_t n, const void *workspace, const int32_t *counts, const int32_t *src,
+                      const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets, uint64_t *stats,
+                      void *stream);
+
+/* Writes each original cell's row to adjacency + offsets[orig].
+ * Replaces gather_adjacency_kernel (adjacency.cuh:7-43). */
+int pg_emit_csr(int64_t n, const void *workspace, const int32_t *counts, const int32_t *offsets, const int32_t *src,
+                const int64_t *src_offsets, int64_t src_stride, int32_t *adjacency, void *stream);
+
+/* Union symmetrisation of an emitted CSR, one phase per call (0: count,
+ * 1: new counts, 2: populate, 3: sort rows); the caller scans new_counts into
+ * new_offsets between phases 1 and 2 and zeroes `extra` before phases 0 and 2.
+ * Replaces count_symmetrized_adjacency_sizes and populate_symmetric_adjacency
+ * (adjacency.cuh:78-159). */
+int pg_symmetrize_union(int64_t n, const int32_t *counts, const int32_t *offsets, const int32_t *adjacency,
+                        int32_t *keep, uint32_t *extra, int32_t *new_counts, const int32_t *new_offsets,
+                        int32_t *new_adjacency, int32_t phase, void *stream);
+
+/* p[0..n) = v (a fill kernel; used instead of hipMemsetAsync). */
+int pg_fill_u32(uint32_t *p, int64_t n, uint32_t v, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARAGRAM_HIP_H */

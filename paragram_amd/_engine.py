@@ -1,0 +1,224 @@
+"""Host driver: runs the HIP stages of libparagram_hip.so on torch's stream.
+
+This is the MI355X counterpart of the reference's C++ orchestration
+``build_voronoi_ultra`` (paragram/csrc/laguerre/voronoi_ultra.cu:143-376) and
+``build_laguerre_voronoi_ultra`` (laguerre_ultra.cu:176-454): index build,
+chunked cell construction, CSR assembly.  All device memory comes from the
+torch caching allocator; all work is ordered on
+``torch.cuda.current_stream()`` of the points' device; the only
+device-to-host read is one 128-byte counter block that carries E.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+
+import torch
+
+from ._chunking import plan_chunks
+
+NUM_STATS = 16
+STAT_NAMES = {
+    0: "fp64_rechecks", 1: "unresolved", 2: "inconsistent", 3: "overflows", 4: "clips",
+    5: "candidates", 6: "nodes", 7: "cells", 8: "asymmetric", 9: "dropped", 10: "leaves",
+    11: "skipped", 15: "edges",
+}
+CELL_BLOCK = 64  # cells per admission block (one wavefront)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparagram_hip.so")
+
+
+class HipLibrary:
+    """ctypes binding of include/paragram_hip.h."""
+
+    def __init__(self, path: str = LIB_PATH):
+        self.path = path
+        self.lib = lib = ctypes.CDLL(path)
+        i64, i32, u64p, vp = ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p
+        sig = {
+            "pg_abi_version": ([], ctypes.c_int),
+            "pg_last_error": ([], ctypes.c_char_p),
+            "pg_row_capacity": ([], ctypes.c_int),
+            "pg_index_workspace_bytes": ([i64, u64p], ctypes.c_int),
+            "pg_build_index": ([i64, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
+            "pg_default_waves": ([ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+            "pg_cell_scratch_bytes": ([i32, i64, u64p], ctypes.c_int),
+            "pg_row_buffer_elems": ([i64, u64p], ctypes.c_int),
+            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp],
+                                 ctypes.c_int),
+            "pg_scan_tmp_bytes": ([i64, u64p], ctypes.c_int),
+            "pg_scan_counts": ([i64, vp, vp, vp, vp], ctypes.c_int),
+            "pg_chunk_counts": ([i64, vp, i64, i64, vp, vp, vp], ctypes.c_int),
+            "pg_pack_rows": ([i64, vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_check_symmetry": ([i64, vp, vp, vp, vp, i64, vp, vp, vp], ctypes.c_int),
+            "pg_emit_csr": ([i64, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
+            "pg_symmetrize_union": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], ctypes.c_int),
+            "pg_fill_u32": ([vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        if lib.pg_abi_version() != 1:
+            raise ImportError("libparagram_hip.so ABI version mismatch")
+
+    def call(self, name: str, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != 0:
+            msg = self.lib.pg_last_error().decode(errors="replace")
+            raise RuntimeError(msg if msg.startswith("Kernel failed") else f"{name}: {msg}")
+
+    def u64(self, name: str, *args) -> int:
+        out = ctypes.c_uint64(0)
+        self.call(name, *args, ctypes.byref(out))
+        return int(out.value)
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Instrument:
+    """Optional per-call measurement hooks (events on the launch stream)."""
+
+    def __init__(self):
+        self.cell_events = []  # (start, end) torch.cuda.Event pairs around each cell launch
+        self.stats = None
+        self.plan = None
+
+
+def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
+                  initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
+                  instrument: Instrument | None = None):
+    """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device."""
+    n = int(points.shape[0])
+    dev = points.device
+    i32 = dict(dtype=torch.int32, device=dev)
+    if n == 0:  # the reference launches a zero-size grid here; return the empty diagram
+        return torch.empty(0, **i32), torch.zeros(1, **i32), torch.empty(0, **i32)
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev)
+        sp = ctypes.c_void_p(stream.cuda_stream)
+        if debug:
+            torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        ws_bytes = lib.u64("pg_index_workspace_bytes", n)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        lib.call("pg_build_index", n, _p(points), _p(weights), int(leaf_size), _p(ws), ws_bytes, sp)
+        if debug:
+            torch.cuda.synchronize(dev)
+            print("BVH Creation time: %f ms" % ((time.perf_counter() - t0) * 1e3))
+        t1 = time.perf_counter()
+
+        waves = ctypes.c_int32(0)
+        lib.call("pg_default_waves", ctypes.byref(waves))
+        waves = int(waves.value)
+        stats = torch.zeros(NUM_STATS, dtype=torch.int64, device=dev)
+        counts = torch.empty(n, **i32)
+        status = torch.empty(n, **i32)
+        row_offsets = torch.empty(n, dtype=torch.int64, device=dev)
+        rowcap = int(lib.lib.pg_row_capacity())
+
+        guesses = initial_guesses
+        goffs = initial_guesses_offsets
+        if guesses is not None:
+            # the kernel indexes guesses through the offsets: reject malformed CSR
+            ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= guesses.numel())).item())
+            if not ok:
+                raise ValueError("initial_guesses_offsets must be non-decreasing and index into initial_guesses")
+
+        total_blocks = (n + CELL_BLOCK - 1) // CELL_BLOCK
+        full_bytes = 4 * lib.u64("pg_row_buffer_elems", total_blocks * CELL_BLOCK)
+        free_bytes, _ = torch.cuda.mem_get_info(dev)
+        plan = plan_chunks(total_blocks, CELL_BLOCK, full_bytes, free_bytes)
+        chunk_cells = min(plan.chunk_threads, n)
+        scratch_bytes = lib.u64("pg_cell_scratch_bytes", waves, chunk_cells)
+        scratch = torch.empty(scratch_bytes, dtype=torch.uint8, device=dev)
+        rows = torch.empty(lib.u64("pg_row_buffer_elems", chunk_cells), **i32)
+        if debug:
+            print("Row slab: %.2f MB/chunk for %d cells, %d chunk(s) over %d points"
+                  % (rows.numel() * 4 / 2**20, chunk_cells, plan.num_chunks, n))
+
+        def cells(begin, end):
+            ev = None
+            if instrument is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(stream)
+            lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, _p(rows),
+                     _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves, sp)
+            if ev is not None:
+                ev[1].record(stream)
+                instrument.cell_events.append(ev)
+
+        scan_tmp = torch.empty(max(lib.u64("pg_scan_tmp_bytes", n + 1), 4), dtype=torch.uint8, device=dev)
+        if plan.num_chunks == 1:
+            cells(0, n)
+            src, src_off = rows, row_offsets
+        else:
+            src_off = torch.empty(n, dtype=torch.int64, device=dev)
+            parts, base = [], 0
+            for c in range(plan.num_chunks):
+                begin = c * chunk_cells
+                if begin >= n:
+                    break
+                end = min(n, begin + chunk_cells)
+                cells(begin, end)
+                m = end - begin
+                cc = torch.empty(m, **i32)
+                lib.call("pg_chunk_counts", n, _p(ws), begin, end, _p(counts), _p(cc), sp)
+                loc = torch.empty(m + 1, **i32)
+                lib.call("pg_scan_counts", m, _p(cc), _p(loc), _p(scan_tmp), sp)
+                total = int(loc[m].item())  # chunked mode: one read per chunk
+                packed = torch.empty(max(total, 1), **i32)
+                lib.call("pg_pack_rows", n, _p(ws), begin, end, _p(counts), _p(loc), _p(rows), _p(row_offsets),
+                         _p(packed), sp)
+                src_off[begin:end] = loc[:m].to(torch.int64) + base
+                parts.append(packed[:total])
+                base += total
+            src = torch.cat(parts) if base > 0 else torch.empty(1, **i32)
+        stride = 0
+
+        offsets = torch.empty(n + 1, **i32)
+        lib.call("pg_scan_counts", n, _p(counts), _p(offsets), _p(scan_tmp), sp)
+        lib.call("pg_check_symmetry", n, _p(ws), _p(counts), _p(src), _p(src_off), stride, _p(offsets), _p(stats), sp)
+        st = stats.cpu()  # the single device-to-host read: E and the symmetry counters
+        e_total = int(st[15])
+        adjacency = torch.empty(max(e_total, 1), **i32)
+        lib.call("pg_emit_csr", n, _p(ws), _p(counts), _p(offsets), _p(src), _p(src_off), stride,
+                 _p(adjacency), sp)
+        adjacency = adjacency[:e_total]
+        if int(st[8]) or int(st[9]):
+            adjacency, offsets = _union(lib, n, counts, offsets, adjacency, scan_tmp, sp, dev)
+        if debug:
+            torch.cuda.synchronize(dev)
+            name = "Laguerre" if weights is not None else "Voronoi"
+            print("%s Construction time: %f ms" % (name, (time.perf_counter() - t1) * 1e3))
+        if instrument is not None:
+            instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
+            instrument.plan = plan
+        return adjacency, offsets, status
+
+
+def _union(lib, n, counts, offsets, adjacency, scan_tmp, sp, dev):
+    """Union symmetrisation (rare path: only when some edge lacks its reverse)."""
+    i32 = dict(dtype=torch.int32, device=dev)
+    keep = torch.empty(n, **i32)
+    extra = torch.empty(n, **i32)
+    new_counts = torch.empty(n, **i32)
+    new_offsets = torch.empty(n + 1, **i32)
+    lib.call("pg_fill_u32", _p(extra), n, 0, sp)
+    lib.call("pg_symmetrize_union", n, _p(counts), _p(offsets), _p(adjacency), _p(keep), _p(extra), _p(new_counts),
+             None, None, 0, sp)
+    lib.call("pg_symmetrize_union", n, _p(counts), _p(offsets), _p(adjacency), _p(keep), _p(extra), _p(new_counts),
+             None, None, 1, sp)
+    lib.call("pg_scan_counts", n, _p(new_counts), _p(new_offsets), _p(scan_tmp), sp)
+    e2 = int(new_offsets[n].item())
+    new_adj = torch.empty(max(e2, 1), **i32)
+    lib.call("pg_fill_u32", _p(extra), n, 0, sp)
+    lib.call("pg_symmetrize_union", n, _p(counts), _p(offsets), _p(adjacency), _p(keep), _p(extra), _p(new_counts),
+             _p(new_offsets), _p(new_adj), 2, sp)
+    lib.call("pg_symmetrize_union", n, _p(counts), _p(offsets), _p(adjacency), _p(keep), _p(extra), _p(new_counts),
+             _p(new_offsets), _p(new_adj), 3, sp)
+    return new_adj[:e2], new_offsets

@@ -1,0 +1,201 @@
+// extern "C" entry points of libparagram_hip.so (declared in
+// include/paragram_hip.h).  Argument checks and error reporting only; the
+// work is in pg_index.hip, pg_cell.hip and pg_csr.hip.
+#include "../../include/paragram_hip.h"
+#include "pg_common.hpp"
+#include "pg_kernels.hpp"
+
+#include <stdio.h>
+#include <string>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    return fail(2, std::string("Kernel failed: ") + what + ": " + hipGetErrorString(e));
+}
+
+inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+int check_n(int64_t n) {
+    if (n < 0) return fail(1, "number of points must be non-negative");
+    if (n > pg::kMaxPoints) return fail(1, "too many points for one call (limit 67108863)");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_abi_version(void) { return PG_ABI_VERSION; }
+
+const char *pg_last_error(void) { return g_last_error.c_str(); }
+
+int pg_row_capacity(void) { return pg::kRowCap; }
+
+int pg_index_workspace_bytes(int64_t n, uint64_t *bytes) {
+    if (int r = check_n(n)) return r;
+    if (!bytes) return fail(1, "bytes is null");
+    *bytes = pg::make_index_layout(n).total;
+    return 0;
+}
+
+int pg_build_index(int64_t n, const float *points, const float *weights, int32_t leaf_size, void *workspace,
+                   uint64_t workspace_bytes, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n == 0) return 0;
+    if (!points || !workspace) return fail(1, "points and workspace must be non-null");
+    if (workspace_bytes < pg::make_index_layout(n).total) return fail(1, "index workspace too small");
+    int leaf = leaf_size < 0 ? pg::kDefaultLeaf : leaf_size;
+    if (leaf < 1) leaf = 1;
+    if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
+    return check_hip(pg::build_index(n, points, weights, leaf, workspace, as_stream(stream)), "build_index");
+}
+
+int pg_default_waves(int32_t *waves) {
+    if (!waves) return fail(1, "waves is null");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return check_hip(e, "hipGetDevice");
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return check_hip(e, "hipDeviceGetAttribute");
+    *waves = cus * 16;
+    return 0;
+}
+
+int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes) {
+    if (waves < 1) return fail(1, "waves must be positive");
+    if (max_cells < 0) return fail(1, "max_cells must be non-negative");
+    if (!bytes) return fail(1, "bytes is null");
+    *bytes = pg::cell_scratch_bytes((waves + 3) / 4 * 4, pg::kTier1Waves, pg::tier1_capacity(max_cells));
+    return 0;
+}
+
+int pg_row_buffer_elems(int64_t cells, uint64_t *elems) {
+    if (cells < 0 || !elems) return fail(1, "invalid argument");
+    *elems = (uint64_t)pg::row_buffer_elems(cells);
+    return 0;
+}
+
+int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t *rows, int64_t *row_offsets,
+                     int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                     int32_t waves, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
+    if (begin == end) return 0;
+    if (!workspace || !rows || !row_offsets || !counts || !status || !stats || !scratch)
+        return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
+    if ((guesses == nullptr) != (guess_offsets == nullptr))
+        return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
+    if (waves < 1) return fail(1, "waves must be positive");
+    const int w0 = (waves + 3) / 4 * 4;
+    const int64_t big_cap = pg::tier1_capacity(end - begin);
+    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap)) return fail(1, "cell scratch too small");
+    pg::CellArgs a;
+    a.n = n;
+    a.index_ws = workspace;
+    a.begin = begin;
+    a.end = end;
+    a.guesses = guesses;
+    a.guess_offsets = guess_offsets;
+    a.guesses_sorted = guesses_sorted;
+    a.rows = rows;
+    a.row_stride = pg::kRowCap;
+    a.row_offsets = row_offsets;
+    a.counts = counts;
+    a.status = status;
+    a.stats = (unsigned long long *)stats;
+    a.scratch = scratch;
+    a.scratch_bytes = scratch_bytes;
+    a.waves = w0;
+    a.waves1 = pg::kTier1Waves;
+    a.big_cap = big_cap;
+    return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells");
+}
+
+int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes) {
+    if (n < 0 || !bytes) return fail(1, "invalid argument");
+    *bytes = 4 * (uint64_t)pg::scan_tmp_elems(n);
+    return 0;
+}
+
+int pg_scan_counts(int64_t n, const int32_t *counts, int32_t *offsets, void *tmp, void *stream) {
+    if (n < 0 || !offsets || (n > 0 && (!counts || !tmp))) return fail(1, "invalid argument");
+    if (n == 0) return check_hip(pg::fill_u32((uint32_t *)offsets, 1, 0u, as_stream(stream)), "scan");
+    return check_hip(pg::exclusive_scan_u32((const uint32_t *)counts, (uint32_t *)offsets, n, (uint32_t *)tmp,
+                                            (uint32_t *)offsets + n, as_stream(stream)),
+                     "scan");
+}
+
+int pg_chunk_counts(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
+                    int32_t *out, void *stream) {
+    if (begin < 0 || end > n || begin > end || !workspace || !counts || !out) return fail(1, "invalid argument");
+    const pg::IndexLayout L = pg::make_index_layout(n);
+    const uint32_t *perm = (const uint32_t *)((const char *)workspace + L.perm);
+    return check_hip(pg::gather_counts_sorted(counts, perm, begin, end, (uint32_t *)out, as_stream(stream)),
+                     "chunk_counts");
+}
+
+int pg_pack_rows(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
+                 const int32_t *local_offsets, const int32_t *rows, const int64_t *row_offsets, int32_t *packed,
+                 void *stream) {
+    if (begin < 0 || end > n || begin > end || !workspace || !counts || !local_offsets || !rows || !row_offsets ||
+        !packed)
+        return fail(1, "invalid argument");
+    const pg::IndexLayout L = pg::make_index_layout(n);
+    const uint32_t *perm = (const uint32_t *)((const char *)workspace + L.perm);
+    return check_hip(pg::pack_rows(begin, end, perm, counts, (const uint32_t *)local_offsets, rows, row_offsets,
+                                   packed, as_stream(stream)),
+                     "pack_rows");
+}
+
+int pg_check_symmetry(int64_t n, const void *workspace, const int32_t *counts, const int32_t *src,
+                      const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets, uint64_t *stats,
+                      void *stream) {
+    if (n <= 0) return 0;
+    if (!workspace || !counts || !src || !offsets || !stats) return fail(1, "invalid argument");
+    if (!src_offsets && src_stride <= 0) return fail(1, "src_stride must be positive without src_offsets");
+    const pg::IndexLayout L = pg::make_index_layout(n);
+    const uint32_t *inv = (const uint32_t *)((const char *)workspace + L.inv);
+    return check_hip(pg::check_symmetry(n, inv, counts, src, src_offsets, src_stride, offsets,
+                                        (unsigned long long *)stats, as_stream(stream)),
+                     "check_symmetry");
+}
+
+int pg_emit_csr(int64_t n, const void *workspace, const int32_t *counts, const int32_t *offsets, const int32_t *src,
+                const int64_t *src_offsets, int64_t src_stride, int32_t *adjacency, void *stream) {
+    if (n <= 0) return 0;
+    if (!workspace || !counts || !offsets || !src || !adjacency) return fail(1, "invalid argument");
+    if (!src_offsets && src_stride <= 0) return fail(1, "src_stride must be positive without src_offsets");
+    const pg::IndexLayout L = pg::make_index_layout(n);
+    const uint32_t *inv = (const uint32_t *)((const char *)workspace + L.inv);
+    return check_hip(pg::emit_rows(n, inv, counts, offsets, src, src_offsets, src_stride, adjacency,
+                                   as_stream(stream)),
+                     "emit_csr");
+}
+
+int pg_symmetrize_union(int64_t n, const int32_t *counts, const int32_t *offsets, const int32_t *adjacency,
+                        int32_t *keep, uint32_t *extra, int32_t *new_counts, const int32_t *new_offsets,
+                        int32_t *new_adjacency, int32_t phase, void *stream) {
+    if (n <= 0) return 0;
+    if (phase < 0 || phase > 3) return fail(1, "phase must be 0..3");
+    return check_hip(pg::symmetrize_union(n, counts, offsets, adjacency, keep, extra, new_counts, new_offsets,
+                                          new_adjacency, phase, as_stream(stream)),
+                     "symmetrize_union");
+}
+
+int pg_fill_u32(uint32_t *p, int64_t n, uint32_t v, void *stream) {
+    if (n < 0 || (n > 0 && !p)) return fail(1, "invalid argument");
+    return check_hip(pg::fill_u32(p, n, v, as_stream(stream)), "fill");
+}
+
+}  // extern "C"

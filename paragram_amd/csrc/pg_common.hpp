@@ -1,0 +1,200 @@
+// Shared definitions of the paragram_amd HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pg {
+
+// ------------------------------------------------------------------ limits
+// Plane and vertex capacity of one cell, in two tiers.  A cell of F faces has
+// 2F-4 vertices; unreferenced planes are compacted when the plane list fills
+// up.  Tier 0 holds nearly every cell; the few that outgrow it (large sparse
+// cells, e.g. near-range LiDAR returns with 200+ faces) are re-run in tier 1.
+constexpr int kTier0Planes = 96;
+constexpr int kTier0Verts = 192;
+constexpr int kTier0Bits = 8;
+constexpr int kTier1Planes = 512;
+constexpr int kTier1Verts = 1024;
+constexpr int kTier1Bits = 10;
+constexpr int kTier1Row = kTier1Planes;
+constexpr int kRowCap = kTier0Planes;  // tier-0 row stride
+constexpr int kTier1Waves = 256;       // persistent waves of the tier-1 launch
+
+// Tier-1 queue capacity for a launch over m cells.
+__host__ __device__ inline int64_t tier1_capacity(int64_t m) {
+    const int64_t c = m / 32 + 256;
+    return c < m ? c : m;
+}
+// Elements of the row buffer of a launch over m cells.
+__host__ __device__ inline int64_t row_buffer_elems(int64_t m) {
+    return m * kRowCap + tier1_capacity(m) * kTier1Planes;
+}
+constexpr int kStackCap = 64;          // LBVH depth is < 56 for N < 2^26
+constexpr int kMaxLeaf = 32;
+constexpr int kDefaultLeaf = 8;
+constexpr int64_t kMaxPoints = (int64_t(1) << 26) - 1;  // leaf refs keep 26 bits of first index
+constexpr int kWave = 64;
+
+// Status codes of the reference (voronoi_convex_cell.cuh:91-100).
+enum Status : int32_t {
+    kSuccess = 0,
+    kPlaneOverflow = 1,
+    kVertexOverflow = 2,
+    kEmptyCell = 3,
+    kSecurityRadiusNotReached = 4,
+    kInconsistentBoundary = 5,
+};
+
+// Device-side counters, accumulated with one atomic per wave.
+enum StatSlot : int {
+    kStatFp64Rechecks = 0,   // FP32 filter undecided, recomputed in FP64
+    kStatUnresolved = 1,     // FP64 also undecided (decided by the FP64 sign)
+    kStatInconsistent = 2,   // orientation or boundary inconsistency
+    kStatOverflow = 3,       // plane/vertex capacity exceeded
+    kStatClips = 4,          // successful clips
+    kStatCandidates = 5,     // candidates that reached the vertex tests
+    kStatNodes = 6,          // internal nodes visited
+    kStatCells = 7,          // cells processed
+    kStatAsymmetric = 8,     // directed edges without a reverse edge
+    kStatDropped = 9,        // row entries pointing at empty cells
+    kStatLeaves = 10,        // leaves visited
+    kStatSkipped = 11,       // candidates rejected by the support test
+    kNumStats = 16,
+};
+
+// ------------------------------------------------------------------ LBVH
+// Internal node of the Karras hierarchy in traversal form: the boxes of both
+// children sit in the parent (one 64-byte fetch per visited node).  A child
+// reference is an internal node index, or a leaf (bit 31 set) that names a
+// contiguous range of the Morton-sorted points: first << 5 | (count - 1).
+struct alignas(16) Node {
+    float4 lo0;  // child 0: lo.xyz, max weight
+    float4 hi0;  // child 0: hi.xyz, reference (bits)
+    float4 lo1;
+    float4 hi1;
+};
+static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
+
+__host__ __device__ inline uint32_t leaf_ref(uint32_t first, uint32_t count) {
+    return 0x80000000u | (first << 5) | (count - 1u);
+}
+__host__ __device__ inline bool is_leaf(uint32_t ref) { return (ref & 0x80000000u) != 0; }
+__host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return (ref & 0x7fffffffu) >> 5; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return (ref & 31u) + 1u; }
+
+// Header at the start of the index workspace (device memory).
+struct IndexHeader {
+    float raw_lo[3], raw_hi[3];   // exact AABB of the points
+    float box_lo[3], box_hi[3];   // clipping box: (min - 1) - 1e-5, (max + 1) + 1e-5 in FP32
+    uint32_t root_ref;
+    uint32_t n;
+    uint32_t has_weights;
+    uint32_t leaf_size;
+    uint32_t has_duplicates;      // some points coincide in position and weight
+};
+
+// Offsets of the arrays inside the index workspace.
+struct IndexLayout {
+    uint64_t header;     // IndexHeader
+    uint64_t points;     // float4 [n]  Morton-sorted (x, y, z, w)
+    uint64_t perm;       // uint32 [n]  sorted -> original
+    uint64_t inv;        // uint32 [n]  original -> sorted
+    uint64_t nodes;      // Node [max(n-1,1)]
+    uint64_t keys;       // uint32 [n]  sort keys (ping)
+    uint64_t keys_alt;   // uint32 [n]  (pong)
+    uint64_t vals_alt;   // uint32 [n]
+    uint64_t hist;       // uint32 [256 * sort_blocks + 256]
+    uint64_t children;   // uint32 [2*(n-1)]  Karras child codes
+    uint64_t parents;    // uint32 [2n-1]     parent of internal (0..n-2) and leaf (n-1..2n-2)
+    uint64_t ranges;     // uint2  [n-1]      covered range of internal nodes
+    uint64_t boxes;      // float4 [2*(n-1)]  internal node AABB lo/w, hi
+    uint64_t flags;      // uint32 [n-1]      refit arrival counters
+    uint64_t reduce;     // float  [6 * reduce_blocks]
+    uint64_t scan_tmp;   // uint32 block sums of the histogram scan
+    uint64_t dup_head;   // uint32 [n]  first sorted index of the point's duplicate group
+    uint64_t dup_next;   // uint32 [n]  next member of the group (0xFFFFFFFF at the end)
+    uint64_t total;
+    int sort_blocks;
+    int reduce_blocks;
+};
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;                 // items per thread per tile
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kReduceThreads = 256;
+constexpr int kReduceBlocksMax = 1024;
+
+__host__ __device__ inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+inline IndexLayout make_index_layout(int64_t n) {
+    IndexLayout L{};
+    const uint64_t nn = (uint64_t)(n > 0 ? n : 1);
+    const uint64_t ni = nn > 1 ? nn - 1 : 1;
+    L.sort_blocks = (int)((nn + kSortTile - 1) / kSortTile);
+    L.reduce_blocks = (int)((nn + kReduceThreads * 8 - 1) / (kReduceThreads * 8));
+    if (L.reduce_blocks > kReduceBlocksMax) L.reduce_blocks = kReduceBlocksMax;
+    if (L.reduce_blocks < 1) L.reduce_blocks = 1;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t at = o;
+        o = align_up(o + bytes, 256);
+        return at;
+    };
+    L.header = take(sizeof(IndexHeader));
+    L.points = take(16 * nn);
+    L.perm = take(4 * nn);
+    L.inv = take(4 * nn);
+    L.nodes = take(sizeof(Node) * ni);
+    L.keys = take(4 * nn);
+    L.keys_alt = take(4 * nn);
+    L.vals_alt = take(4 * nn);
+    L.hist = take(4 * (256ull * L.sort_blocks + 256));
+    L.children = take(8 * ni);
+    L.parents = take(4 * (2 * nn));
+    L.ranges = take(8 * ni);
+    L.boxes = take(32 * ni);
+    L.flags = take(4 * ni);
+    L.reduce = take(4 * 6 * (uint64_t)L.reduce_blocks);
+    L.scan_tmp = take(4 * ((256ull * L.sort_blocks + 4095) / 4096 + 2));
+    L.dup_head = take(4 * nn);
+    L.dup_next = take(4 * nn);
+    L.total = o;
+    return L;
+}
+
+// ------------------------------------------------------------------ cell scratch
+// Per-lane cell state lives in a scratch buffer owned by the caller, laid out
+// per wave as [slot][lane] so that a wave touching the same slot in every
+// lane reads 64 consecutive elements.
+struct CellScratchLayout {
+    uint64_t plane_ah;    // float4 [kPlaneCap]  (a.x, a.y, a.z, H)
+    uint64_t plane_aux;   // float2 [kPlaneCap]  (permanent of H, owner bits)
+    uint64_t vert_tri;    // uint32 [kVertCap]   packed plane triple
+    uint64_t vert_nd;     // float4 [kVertCap]   homogeneous position (N, D)
+    uint64_t vert_perm;   // float2 [kVertCap]   permanents (max N, D)
+    uint64_t next;        // uint32 [kPlaneCap]  boundary successor, generation tagged
+    uint64_t stack;       // uint32 [kStackCap]  traversal stack
+    uint64_t per_wave;    // bytes per wave
+};
+
+__host__ __device__ inline CellScratchLayout make_cell_scratch_layout(int kPlaneCap, int kVertCap) {
+    CellScratchLayout L{};
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t at = o;
+        o = align_up(o + bytes * kWave, 256);
+        return at;
+    };
+    L.plane_ah = take(16ull * kPlaneCap);
+    L.plane_aux = take(8ull * kPlaneCap);
+    L.vert_tri = take(4ull * kVertCap);
+    L.vert_nd = take(16ull * kVertCap);
+    L.vert_perm = take(8ull * kVertCap);
+    L.next = take(4ull * kPlaneCap);
+    L.stack = take(4ull * kStackCap);
+    L.per_wave = o;
+    return L;
+}
+
+}  // namespace pg

@@ -1,0 +1,420 @@
+// LBVH build for gfx950: exact point AABB and clipping box, 30-bit Morton
+// keys, stable LSD radix sort staged through LDS, Karras hierarchy, and a
+// bottom-up refit that fuses the AABB, the max-weight bound and the leaf
+// collapse into the traversal-form node records.
+//
+// Replaces the reference's cuBQL gpuBuilder + create_boxes_kernel
+// (pwr_bvh.cu:24-39,217-339) and the max-weight propagation kernels
+// (pwr_bvh.cu:42-157).  Leaves name contiguous ranges of the sorted points,
+// so no primitive-id indirection is needed.
+#include "pg_common.hpp"
+#include "pg_kernels.hpp"
+
+#include <float.h>
+
+namespace pg {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint64_t lanemask_lt64() {
+    const uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ------------------------------------------------------------------ AABB
+__global__ void __launch_bounds__(kReduceThreads) bbox_partial_kernel(const float *__restrict__ pts, int64_t n,
+                                                                      float *__restrict__ partial) {
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            const float v = pts[3 * i + k];
+            mn[k] = fminf(mn[k], v);
+            mx[k] = fmaxf(mx[k], v);
+        }
+    }
+    __shared__ float red[6][kReduceThreads];
+    for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = mn[k];
+        red[3 + k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fminf(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+                red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kReduceThreads) bbox_final_kernel(const float *__restrict__ partial, int nparts,
+                                                                    IndexHeader *__restrict__ hdr, int64_t n,
+                                                                    int has_w, int leaf_size) {
+    __shared__ float red[6][kReduceThreads];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], partial[6 * i + k]);
+            mx[k] = fmaxf(mx[k], partial[6 * i + 3 + k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = mn[k];
+        red[3 + k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fminf(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+                red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) {
+            const float lo = red[k][0], hi = red[3 + k][0];
+            hdr->raw_lo[k] = lo;
+            hdr->raw_hi[k] = hi;
+            // voronoi_ultra.cu:237-238 then voronoi_convex_cell.cuh:187-193,
+            // two FP32 roundings in this order.
+            const float a = __fsub_rn(lo, 1.0f);
+            const float b = __fadd_rn(hi, 1.0f);
+            hdr->box_lo[k] = __fsub_rn(a, 1e-5f);
+            hdr->box_hi[k] = __fadd_rn(b, 1e-5f);
+        }
+        hdr->n = (uint32_t)n;
+        hdr->has_weights = (uint32_t)has_w;
+        hdr->leaf_size = (uint32_t)leaf_size;
+        hdr->root_ref = (n > leaf_size) ? 0u : leaf_ref(0u, (uint32_t)n);
+        hdr->has_duplicates = 0u;
+    }
+}
+
+// ------------------------------------------------------------------ Morton
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// 30-bit Morton key of each point on a cubic 1024^3 grid over the AABB (one
+// scale for all axes so flat clouds keep their aspect).
+__global__ void morton_kernel(const float *__restrict__ pts, int64_t n, const IndexHeader *__restrict__ hdr,
+                              uint32_t *__restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float ex = fmaxf(fmaxf(hdr->raw_hi[0] - hdr->raw_lo[0], hdr->raw_hi[1] - hdr->raw_lo[1]),
+                           hdr->raw_hi[2] - hdr->raw_lo[2]);
+    const float scale = ex > 0.0f ? 1024.0f / ex : 0.0f;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float t = (pts[3 * i + k] - hdr->raw_lo[k]) * scale;
+        t = fminf(fmaxf(t, 0.0f), 1023.0f);
+        q[k] = (uint32_t)t;
+    }
+    keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+}
+
+// ------------------------------------------------------------------ radix sort
+// Pass histogram: digit-major counts hist[d * nblocks + b] for the tile of
+// block b.
+__global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(const uint32_t *__restrict__ keys, int64_t n,
+                                                                  int shift, uint32_t *__restrict__ hist, int nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int it = 0; it < kSortItems; ++it) {
+        const int64_t i = base + it * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: item (it, tid) of the tile has position it*256 + tid; its
+// rank among equal digits of the tile is the running count of earlier
+// iterations, plus equal digits of lower waves in this iteration, plus equal
+// digits of lower lanes in its own wave (8 ballots give the peer mask).
+__global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, int64_t n, int shift,
+    const uint32_t *__restrict__ hist_scanned, int nblocks, uint32_t *__restrict__ keys_out,
+    uint32_t *__restrict__ vals_out) {
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[kWaves][256];
+    __shared__ uint32_t gbase[256];
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    run[tid] = 0;
+    gbase[tid] = hist_scanned[(int64_t)tid * nblocks + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const uint64_t lt = lanemask_lt64();
+    for (int it = 0; it < kSortItems; ++it) {
+        for (int w = 0; w < kWaves; ++w) wc[w][tid] = 0;
+        __syncthreads();
+        const int64_t i = base + it * kSortThreads + tid;
+        const bool valid = i < n;
+        uint32_t key = 0, val = 0, d = 0;
+        if (valid) {
+            key = keys_in[i];
+            val = vals_in ? vals_in[i] : (uint32_t)i;
+            d = (key >> shift) & 255u;
+        }
+        uint64_t peers = wave_ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = wave_ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wc[w][d];
+            const uint32_t dst = gbase[d] + pos;
+            if ((int64_t)dst < n) {
+                keys_out[dst] = key;
+                vals_out[dst] = val;
+            }
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < kWaves; ++w) add += wc[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ reorder
+__global__ void reorder_kernel(const float *__restrict__ pts, const float *__restrict__ w, int64_t n,
+                               const uint32_t *__restrict__ perm, float4 *__restrict__ sorted,
+                               uint32_t *__restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    if ((int64_t)j >= n) return;
+    sorted[i] = make_float4(pts[3 * (int64_t)j], pts[3 * (int64_t)j + 1], pts[3 * (int64_t)j + 2], w ? w[j] : 0.0f);
+    inv[j] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------------ Karras
+__device__ __forceinline__ int karras_delta(const uint32_t *__restrict__ keys, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// T. Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
+// and k-d Trees", HPG 2012: internal node i finds its range by exponential
+// and binary search on the common-prefix length, then its split.
+__global__ void karras_kernel(const uint32_t *__restrict__ keys, int64_t n, uint32_t *__restrict__ children,
+                              uint32_t *__restrict__ parents, uint2 *__restrict__ ranges) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = karras_delta(keys, n, i, i - d);
+    int64_t lmax = 2;
+    while (karras_delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (karras_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = karras_delta(keys, n, i, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (karras_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int64_t split = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t first = i < j ? i : j, last = i < j ? j : i;
+    uint32_t left, right;
+    if (first == split) {
+        left = 0x80000000u | (uint32_t)split;
+        parents[(n - 1) + split] = (uint32_t)i;
+    } else {
+        left = (uint32_t)split;
+        parents[split] = (uint32_t)i;
+    }
+    if (last == split + 1) {
+        right = 0x80000000u | (uint32_t)(split + 1);
+        parents[(n - 1) + split + 1] = (uint32_t)i;
+    } else {
+        right = (uint32_t)(split + 1);
+        parents[split + 1] = (uint32_t)i;
+    }
+    children[2 * i] = left;
+    children[2 * i + 1] = right;
+    ranges[i] = make_uint2((uint32_t)first, (uint32_t)last);
+    if (i == 0) parents[0] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ float load_coherent(const float *p) {
+    return __uint_as_float(__hip_atomic_load((const uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ uint32_t load_coherent_u(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up refit: one thread per sorted point climbs; the second arrival at
+// a node (arrival counter) builds the node's traversal record from both
+// children and its AABB / max weight.
+__global__ void refit_kernel(const float4 *__restrict__ pts, int64_t n, const uint32_t *__restrict__ children,
+                             const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
+                             float4 *__restrict__ boxes, uint32_t *__restrict__ flags, Node *__restrict__ nodes,
+                             int leaf_size) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t p = parents[(n - 1) + t];
+    // the hierarchy is at most 30 + 26 levels deep; the cap only bounds the loop
+    for (int level = 0; level < 128 && p != 0xFFFFFFFFu && (int64_t)p < n - 1; ++level) {
+        __threadfence();
+        const uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;
+        __threadfence();
+        float4 lo[2], hi[2];
+        uint32_t ref[2];
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t ch = children[2 * p + c];
+            uint32_t first, last;
+            if (ch & 0x80000000u) {
+                const uint32_t prim = ch & 0x7fffffffu;
+                const float4 q = pts[prim];
+                lo[c] = q;
+                hi[c] = make_float4(q.x, q.y, q.z, 0.0f);
+                first = last = prim;
+            } else {
+                const float *b = (const float *)&boxes[2 * (int64_t)ch];
+                lo[c] = make_float4(load_coherent(b + 0), load_coherent(b + 1), load_coherent(b + 2), load_coherent(b + 3));
+                hi[c] = make_float4(load_coherent(b + 4), load_coherent(b + 5), load_coherent(b + 6), 0.0f);
+                const uint2 r = ranges[ch];
+                first = r.x;
+                last = r.y;
+            }
+            const uint32_t count = last - first + 1;
+            ref[c] = (count <= (uint32_t)leaf_size) ? leaf_ref(first, count) : ch;
+        }
+        Node nd;
+        nd.lo0 = lo[0];
+        nd.hi0 = make_float4(hi[0].x, hi[0].y, hi[0].z, __uint_as_float(ref[0]));
+        nd.lo1 = lo[1];
+        nd.hi1 = make_float4(hi[1].x, hi[1].y, hi[1].z, __uint_as_float(ref[1]));
+        nodes[p] = nd;
+        boxes[2 * (int64_t)p] = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z),
+                                            fmaxf(lo[0].w, lo[1].w));
+        boxes[2 * (int64_t)p + 1] =
+            make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
+        p = parents[p];
+    }
+}
+
+// Points identical in position and weight have identical Morton keys, so
+// each duplicate group lies inside one run of equal keys: scan the run.
+__global__ void duplicates_kernel(const uint32_t *__restrict__ keys, const float4 *__restrict__ pts, int64_t n,
+                                  uint32_t *__restrict__ head, uint32_t *__restrict__ next,
+                                  IndexHeader *__restrict__ hdr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    const float4 p = pts[i];
+    int64_t h = i;
+    for (int64_t j = i - 1; j >= 0 && keys[j] == k; --j) {
+        const float4 q = pts[j];
+        if (q.x == p.x && q.y == p.y && q.z == p.z && q.w == p.w) h = j;
+    }
+    uint32_t nx = 0xFFFFFFFFu;
+    for (int64_t j = i + 1; j < n && keys[j] == k; ++j) {
+        const float4 q = pts[j];
+        if (q.x == p.x && q.y == p.y && q.z == p.z && q.w == p.w) {
+            nx = (uint32_t)j;
+            break;
+        }
+    }
+    head[i] = (uint32_t)h;
+    next[i] = nx;
+    if (h != i || nx != 0xFFFFFFFFu) hdr->has_duplicates = 1u;
+}
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ host
+static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+hipError_t fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
+                       hipStream_t stream) {
+    const IndexLayout L = make_index_layout(n);
+    char *base = (char *)ws;
+    IndexHeader *hdr = (IndexHeader *)(base + L.header);
+    float4 *sorted = (float4 *)(base + L.points);
+    uint32_t *perm = (uint32_t *)(base + L.perm);
+    uint32_t *inv = (uint32_t *)(base + L.inv);
+    Node *nodes = (Node *)(base + L.nodes);
+    uint32_t *keys = (uint32_t *)(base + L.keys);
+    uint32_t *keys_alt = (uint32_t *)(base + L.keys_alt);
+    uint32_t *vals_alt = (uint32_t *)(base + L.vals_alt);
+    uint32_t *hist = (uint32_t *)(base + L.hist);
+    uint32_t *children = (uint32_t *)(base + L.children);
+    uint32_t *parents = (uint32_t *)(base + L.parents);
+    uint2 *ranges = (uint2 *)(base + L.ranges);
+    float4 *boxes = (float4 *)(base + L.boxes);
+    uint32_t *flags = (uint32_t *)(base + L.flags);
+    float *reduce = (float *)(base + L.reduce);
+    uint32_t *scan_tmp = (uint32_t *)(base + L.scan_tmp);
+    uint32_t *dup_head = (uint32_t *)(base + L.dup_head);
+    uint32_t *dup_next = (uint32_t *)(base + L.dup_next);
+
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(L.reduce_blocks), dim3(kReduceThreads), 0, stream, points, n, reduce);
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kReduceThreads), 0, stream, reduce, L.reduce_blocks, hdr, n,
+                       weights ? 1 : 0, leaf_size);
+    hipLaunchKernelGGL(morton_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, hdr, keys);
+
+    // 4 stable passes of 8 bits: keys -> alt -> keys -> alt -> keys
+    const int nb = L.sort_blocks;
+    const uint32_t *kin = keys;
+    const uint32_t *vin = nullptr;
+    for (int pass = 0; pass < 4; ++pass) {
+        uint32_t *kout = (pass % 2 == 0) ? keys_alt : keys;
+        uint32_t *vout = (pass % 2 == 0) ? vals_alt : perm;
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, stream, kin, n, shift, hist, nb);
+        hipError_t e = exclusive_scan_u32(hist, hist, (int64_t)256 * nb, scan_tmp, hist + (int64_t)256 * nb, stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, stream, kin, vin, n, shift, hist, nb,
+                           kout, vout);
+        kin = kout;
+        vin = vout;
+    }
+    hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, weights, n, perm, sorted,
+                       inv);
+    hipLaunchKernelGGL(duplicates_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, keys, sorted, n, dup_head,
+                       dup_next, hdr);
+    if (n >= 2) {
+        hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, keys, n, children, parents,
+                           ranges);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, flags, n - 1, 0u);
+        hipLaunchKernelGGL(refit_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted, n, children, parents,
+                           ranges, boxes, flags, nodes, leaf_size);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pg

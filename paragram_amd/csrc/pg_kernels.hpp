@@ -1,0 +1,56 @@
+// Host-side launchers shared between the translation units of the library.
+#pragma once
+
+#include "pg_common.hpp"
+
+namespace pg {
+
+// Exclusive scan of n uint32 values (in place allowed).  tmp holds
+// scan_tmp_elems(n) uint32; total (device, may be null) receives the sum.
+int64_t scan_tmp_elems(int64_t n);
+hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, uint32_t *total,
+                              hipStream_t stream);
+
+hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
+                       hipStream_t stream);
+
+struct CellArgs {
+    int64_t n;
+    const void *index_ws;
+    int64_t begin, end;           // sorted cell range
+    const int32_t *guesses;       // may be null
+    const int32_t *guess_offsets; // indexed by original id (null with guesses == null)
+    int guesses_sorted;           // guesses already hold sorted indices (kNN output)
+    int32_t *rows;                // (end - begin) x row_stride, then big_cap x kTier1Row
+    int row_stride;
+    int64_t *row_offsets;         // [n] sorted cell -> row start in rows
+    int32_t *counts;              // [n] original order
+    int32_t *status;              // [n] original order
+    unsigned long long *stats;    // kNumStats counters (accumulated)
+    void *scratch;
+    uint64_t scratch_bytes;
+    int waves;                    // persistent tier-0 waves (scratch holds one slot per wave)
+    int waves1;                   // persistent tier-1 waves
+    int64_t big_cap;              // tier-1 queue capacity
+};
+hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
+uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap);
+
+// CSR helpers.
+hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int64_t begin, int64_t end,
+                                uint32_t *out, hipStream_t stream);
+hipError_t emit_rows(int64_t n, const uint32_t *inv, const int32_t *counts, const int32_t *offsets,
+                     const int32_t *src, const int64_t *src_offsets, int64_t src_stride, int32_t *adjacency,
+                     hipStream_t stream);
+hipError_t check_symmetry(int64_t n, const uint32_t *inv, const int32_t *counts, const int32_t *src,
+                          const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets,
+                          unsigned long long *stats, hipStream_t stream);
+hipError_t pack_rows(int64_t begin, int64_t end, const uint32_t *perm, const int32_t *counts,
+                     const uint32_t *local_offsets, const int32_t *rows, const int64_t *row_offsets,
+                     int32_t *packed, hipStream_t stream);
+hipError_t symmetrize_union(int64_t n, const int32_t *counts, const int32_t *offsets, const int32_t *adj,
+                            int32_t *keep, uint32_t *extra, int32_t *new_counts, const int32_t *new_offsets,
+                            int32_t *new_adj, int phase, hipStream_t stream);
+hipError_t fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t stream);
+
+}  // namespace pg
