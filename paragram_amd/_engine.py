@@ -46,7 +46,7 @@ class HipLibrary:
             "pg_default_waves": ([ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
             "pg_cell_scratch_bytes": ([i32, i64, u64p], ctypes.c_int),
             "pg_row_buffer_elems": ([i64, u64p], ctypes.c_int),
-            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp],
+            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp, vp],
                                  ctypes.c_int),
             "pg_scan_tmp_bytes": ([i64, u64p], ctypes.c_int),
             "pg_scan_counts": ([i64, vp, vp, vp, vp], ctypes.c_int),
@@ -87,6 +87,7 @@ class Instrument:
         self.cell_events = []  # (start, end) torch.cuda.Event pairs around each cell launch
         self.stats = None
         self.plan = None
+        self.cycles = None     # set to an int32 tensor [n] to collect per-cell clock cycles
 
 
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
@@ -147,7 +148,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(stream)
             lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, _p(rows),
-                     _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves, sp)
+                     _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves,
+                     _p(instrument.cycles if instrument is not None else None), sp)
             if ev is not None:
                 ev[1].record(stream)
                 instrument.cell_events.append(ev)

@@ -88,7 +88,7 @@ int pg_row_buffer_elems(int64_t cells, uint64_t *elems) {
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                      const int32_t *guess_offsets, int32_t guesses_sorted, int32_t *rows, int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                     int32_t waves, void *stream) {
+                     int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
     if (begin == end) return 0;
@@ -119,6 +119,7 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
     a.waves = w0;
     a.waves1 = pg::kTier1Waves;
     a.big_cap = big_cap;
+    a.cycles = cell_cycles;
     return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells");
 }
 

@@ -64,6 +64,7 @@ struct CellParams {
     uint64_t per_wave1;
     uint32_t *big_list;           // tier-1 queue of sorted cell ids
     uint32_t big_cap;
+    uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
 };
 
 // A vertex is a triple of plane slots packed kBits bits apiece.
@@ -105,6 +106,7 @@ struct Cell {
     float sx, sy, sz, sw;
     int nv, np;
     float L[3], U[3];
+    float R8[8];  // squared distance to the AABB corner of each octant (bit k: +axis k)
     bool bounded;
     int status;
     uint32_t gen;
@@ -302,6 +304,12 @@ struct Cell {
         bounded = ok;
         L[0] = lo0, L[1] = lo1, L[2] = lo2;
         U[0] = hi0, U[1] = hi1, U[2] = hi2;
+        // corner radii (voronoi_bvh.cuh:40-66), rounded up by the slack
+        const float l0 = lo0 * lo0, l1 = lo1 * lo1, l2 = lo2 * lo2;
+        const float u0 = hi0 * hi0, u1 = hi1 * hi1, u2 = hi2 * hi2;
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            R8[o] = ((o & 1 ? u0 : l0) + (o & 2 ? u1 : l1) + (o & 4 ? u2 : l2)) * (1.0f + kSlack);
     }
 
     // Can no point plane with normal a and offset H (permanent pH) cut the
@@ -317,6 +325,15 @@ struct Cell {
     // Node prune: H >= d_min^2 - (w_max - w_s) for every point of the box,
     // and the support of the AABB over the box is separable per axis, so its
     // maximum is the sum of per-axis maxima over the box's two faces.
+    // Node prune.  Every point p of the box has a = p - s with |a| >= d and
+    // H = |a|^2 - (w_p - w_s) >= |a|^2 - dw (dw = w_max - w_s).  Two bounds
+    // on the cell side, either of which suffices:
+    //  * octant radius (voronoi_bvh.cuh:71-107, pwr_bvh.cuh:150-185): a . y
+    //    <= |a| rho with rho the largest AABB corner radius over the octants
+    //    the box overlaps, so no cut when |a|^2 - rho |a| - dw >= 0 for all
+    //    |a| >= d, i.e. 2d >= rho and (d^2 - dw)^2 >= rho^2 d^2, d^2 > dw;
+    //  * support: a . y <= sum_k max over the box's faces of a_k L_k, a_k U_k
+    //    (separable per axis), against the lower bound d^2 - dw of H.
     __device__ __forceinline__ bool node_prunable(float4 lo, float4 hi, float &key) const {
         const float alx = lo.x - sx, aly = lo.y - sy, alz = lo.z - sz;
         const float ahx = hi.x - sx, ahy = hi.y - sy, ahz = hi.z - sz;
@@ -324,9 +341,24 @@ struct Cell {
         const float dy = fmaxf(fmaxf(aly, -ahy), 0.0f);
         const float dz = fmaxf(fmaxf(alz, -ahz), 0.0f);
         const float d2 = dx * dx + dy * dy + dz * dz;
-        const float hlow = d2 - (lo.w - sw);
+        const float dw = lo.w - sw;
+        const float hlow = d2 - dw;
         key = hlow;
         if (!bounded) return false;
+        // octant radius
+        const bool nx_ = alx <= 0.0f, px_ = ahx >= 0.0f;
+        const bool ny_ = aly <= 0.0f, py_ = ahy >= 0.0f;
+        const bool nz_ = alz <= 0.0f, pz_ = ahz >= 0.0f;
+        float rho2 = 0.0f;
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const bool in = ((o & 1) ? px_ : nx_) && ((o & 2) ? py_ : ny_) && ((o & 4) ? pz_ : nz_);
+            rho2 = in ? fmaxf(rho2, R8[o]) : rho2;
+        }
+        const float d2lo = d2 * (1.0f - kSlack);
+        const float h = d2lo - dw - kSlack * (fabsf(lo.w) + fabsf(sw)) - kTiny32;
+        if (4.0f * d2lo > rho2 && h > 0.0f && h * h > rho2 * d2 * (1.0f + kSlack)) return true;
+        // support
         const float S = fmaxf(fmaxf(alx * L[0], alx * U[0]), fmaxf(ahx * L[0], ahx * U[0])) +
                         fmaxf(fmaxf(aly * L[1], aly * U[1]), fmaxf(ahy * L[1], ahy * U[1])) +
                         fmaxf(fmaxf(alz * L[2], alz * U[2]), fmaxf(ahz * L[2], ahz * U[2]));
@@ -384,6 +416,19 @@ struct Cell {
     // Returns true when the cell changed.
     __device__ bool clip(uint32_t owner, float ax, float ay, float az, float H, float pH) {
         const float pa1 = fabsf(ax) + fabsf(ay) + fabsf(az);
+        // Read-only sweep first (independent loads, no calls): most offered
+        // planes remove nothing and stop here.
+        bool hit = false;
+#pragma unroll 4
+        for (int v = 0; v < nv; ++v) {
+            const float4 nd = vn(v);
+            const float2 pp = vp(v);
+            const float Pv = ax * nd.x + ay * nd.y + az * nd.z - H * nd.w;
+            const float pP = pa1 * pp.x + pH * pp.y;
+            const bool dec = fabsf(Pv) > kG32 * pP && pP > kTiny32;
+            hit |= !dec || Pv > 0.0f;
+        }
+        if (!hit) return false;
         // partition: vertices strictly beyond the plane move to the tail
         int R = 0, i = 0;
         while (i < nv - R) {
@@ -704,39 +749,11 @@ __device__ void flush_stats(const Cell<kPCap, kVCap, kBits> &c, const CellParams
     }
 }
 
-// Capacity tier 0: every cell of [begin, end).  A cell that outgrows the
-// tier is queued for tier 1 (when the queue has room) instead of emitted.
-__global__ void __launch_bounds__(256) cell_kernel(const CellParams prm) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    Cell<kTier0Planes, kTier0Verts, kTier0Bits> c;
-    setup_cell(c, prm, prm.scratch + wave * prm.per_wave0, lane);
-    uint32_t cells = 0;
-    const int64_t count = prm.end - prm.begin;
-    while (true) {
-        uint32_t c0 = 0;
-        if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kWave);
-        c0 = __shfl(c0, 0);
-        if ((int64_t)c0 >= count) break;
-        const int64_t cell = prm.begin + (int64_t)c0 + lane;
-        if (cell < prm.end) {
-            const int64_t local = cell - prm.begin;
-            int32_t *row = prm.rows + local * prm.row_stride;
-            prm.row_offsets[cell] = local * prm.row_stride;
-            if (!c.run((uint32_t)cell, prm, row, prm.row_stride, false)) {
-                const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
-                if (slot < prm.big_cap) {
-                    prm.big_list[slot] = (uint32_t)cell;
-                } else {
-                    c.c_overflow++;
-                    c.run((uint32_t)cell, prm, row, prm.row_stride, true);
-                }
-            }
-            cells++;
-        }
-    }
-    flush_stats(c, prm, cells, lane);
-}
+}  // namespace pg
+
+#include "pg_group_cell.hpp"
+
+namespace pg {
 
 // Capacity tier 1: the queued cells, rows written after the tier-0 slab.
 __global__ void __launch_bounds__(256) cell_big_kernel(const CellParams prm) {
@@ -755,7 +772,9 @@ __global__ void __launch_bounds__(256) cell_big_kernel(const CellParams prm) {
             const uint32_t cell = prm.big_list[k];
             const int64_t off = (int64_t)(prm.end - prm.begin) * prm.row_stride + (int64_t)k * kTier1Row;
             prm.row_offsets[cell] = off;
+            const uint64_t t0 = clock64();
             if (!c.run(cell, prm, prm.rows + off, kTier1Row, true)) c.c_overflow++;
+            if (prm.cycles) prm.cycles[prm.perm[cell]] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
         }
     }
     flush_stats(c, prm, 0, lane);
@@ -766,9 +785,9 @@ __global__ void cell_prep_kernel(uint32_t *counters) {
 }
 
 uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap) {
-    const CellScratchLayout S0 = make_cell_scratch_layout(kTier0Planes, kTier0Verts);
+    (void)waves;  // tier 0 keeps its cells in LDS
     const CellScratchLayout S1 = make_cell_scratch_layout(kTier1Planes, kTier1Verts);
-    return 256 + S0.per_wave * (uint64_t)waves + S1.per_wave * (uint64_t)waves1 + 4 * (uint64_t)big_cap;
+    return 256 + S1.per_wave * (uint64_t)waves1 + 4 * (uint64_t)big_cap;
 }
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
@@ -795,23 +814,23 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.counts = a.counts;
     prm.status = a.status;
     prm.stats = a.stats;
-    const CellScratchLayout S0 = make_cell_scratch_layout(kTier0Planes, kTier0Verts);
+    prm.cycles = a.cycles;
     const CellScratchLayout S1 = make_cell_scratch_layout(kTier1Planes, kTier1Verts);
     char *sc = (char *)a.scratch;
     prm.counters = (uint32_t *)sc;
-    prm.scratch = sc + 256;
-    prm.per_wave0 = S0.per_wave;
-    prm.scratch1 = prm.scratch + S0.per_wave * (uint64_t)a.waves;
+    prm.scratch = nullptr;
+    prm.per_wave0 = 0;
+    prm.scratch1 = sc + 256;
     prm.per_wave1 = S1.per_wave;
     prm.big_list = (uint32_t *)(prm.scratch1 + S1.per_wave * (uint64_t)a.waves1);
     prm.big_cap = (uint32_t)a.big_cap;
     const int64_t count = a.end - a.begin;
     if (count <= 0) return hipSuccess;
     if (cell_scratch_bytes(a.waves, a.waves1, a.big_cap) > a.scratch_bytes) return hipErrorInvalidValue;
-    const int64_t waves_needed = (count + kWave - 1) / kWave;
-    const int64_t waves = a.waves < waves_needed ? a.waves : waves_needed;
+    const int64_t blocks_needed = (count + kGCellsPerBlock - 1) / kGCellsPerBlock;
+    const int64_t blocks = a.waves / 4 < blocks_needed ? a.waves / 4 : blocks_needed;
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
-    hipLaunchKernelGGL(cell_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, prm);
+    hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, stream, prm);
     hipLaunchKernelGGL(cell_big_kernel, dim3((unsigned)(a.waves1 / 4)), dim3(256), 0, stream, prm);
     return hipGetLastError();
 }

@@ -18,7 +18,7 @@ constexpr int kTier1Planes = 512;
 constexpr int kTier1Verts = 1024;
 constexpr int kTier1Bits = 10;
 constexpr int kTier1Row = kTier1Planes;
-constexpr int kRowCap = kTier0Planes;  // tier-0 row stride
+constexpr int kRowCap = 48;           // tier-0 row stride (= the tier-0 plane capacity)
 constexpr int kTier1Waves = 256;       // persistent waves of the tier-1 launch
 
 // Tier-1 queue capacity for a launch over m cells.

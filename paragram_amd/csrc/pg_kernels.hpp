@@ -32,6 +32,7 @@ struct CellArgs {
     int waves;                    // persistent tier-0 waves (scratch holds one slot per wave)
     int waves1;                   // persistent tier-1 waves
     int64_t big_cap;              // tier-1 queue capacity
+    uint32_t *cycles;             // optional per-cell cost counters
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
 uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap);
