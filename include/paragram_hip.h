@@ -86,13 +86,17 @@ int pg_row_buffer_elems(int64_t cells, uint64_t *elems);
  * rows holds pg_row_buffer_elems(end - begin) elements.
  * guesses / guess_offsets: optional CSR of candidate ids clipped first
  * (original ids, or sorted ids when guesses_sorted != 0); negative ids skip.
+ * seed_window (0..16): without guesses, that many Morton-adjacent points are
+ * offered first (replaces the kNN pre-clip of voronoi_ultra.cu:70-85;
+ * changes speed only, never the result).
  * stats: PG_NUM_STATS counters, accumulated.
  * cell_cycles: optional (may be null) per-cell cost in GPU clock cycles,
  * original order; bit 31 set marks cells built by the large-capacity tier.
  * Replaces the chunk loop and compute_bvh_{voronoi,power_diagram}_kernel
  * (voronoi_ultra.cu:39-141,283-305; laguerre_ultra.cu:43-174,337-357). */
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
-                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t *rows, int64_t *row_offsets,
+                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window, int32_t *rows,
+                     int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream);
 

@@ -46,7 +46,8 @@ class HipLibrary:
             "pg_default_waves": ([ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
             "pg_cell_scratch_bytes": ([i32, i64, u64p], ctypes.c_int),
             "pg_row_buffer_elems": ([i64, u64p], ctypes.c_int),
-            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp, vp],
+            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp,
+                                  vp],
                                  ctypes.c_int),
             "pg_scan_tmp_bytes": ([i64, u64p], ctypes.c_int),
             "pg_scan_counts": ([i64, vp, vp, vp, vp], ctypes.c_int),
@@ -124,6 +125,9 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
 
         guesses = initial_guesses
         goffs = initial_guesses_offsets
+        # knn > 0: clip by the 16 Morton-adjacent points before the traversal
+        # (initial guesses replace it, as they replace kNN, voronoi_ultra.cu:158-169)
+        seed_window = 16 if (knn > 0 and guesses is None) else 0
         if guesses is not None:
             # the kernel indexes guesses through the offsets: reject malformed CSR
             ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= guesses.numel())).item())
@@ -147,7 +151,7 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             if instrument is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(stream)
-            lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, _p(rows),
+            lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, seed_window, _p(rows),
                      _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves,
                      _p(instrument.cycles if instrument is not None else None), sp)
             if ev is not None:

@@ -86,7 +86,8 @@ int pg_row_buffer_elems(int64_t cells, uint64_t *elems) {
 }
 
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
-                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t *rows, int64_t *row_offsets,
+                     const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window, int32_t *rows,
+                     int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
@@ -108,6 +109,7 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
     a.guesses = guesses;
     a.guess_offsets = guess_offsets;
     a.guesses_sorted = guesses_sorted;
+    a.seed_window = seed_window < 0 ? 0 : (seed_window > 16 ? 16 : seed_window);
     a.rows = rows;
     a.row_stride = pg::kRowCap;
     a.row_offsets = row_offsets;

@@ -51,6 +51,7 @@ struct CellParams {
     const int32_t *guesses;
     const int32_t *guess_offsets;
     int guesses_sorted;
+    int seed_window;              // Morton neighbours offered first (0: none)
     int32_t *rows;                // tier-0 slab (row_stride per cell) then tier-1 rows
     int row_stride;
     int64_t *row_offsets;         // [n] sorted cell -> row start in `rows`
@@ -808,6 +809,7 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.guesses = a.guesses;
     prm.guess_offsets = a.guess_offsets;
     prm.guesses_sorted = a.guesses_sorted;
+    prm.seed_window = a.seed_window;
     prm.rows = a.rows;
     prm.row_stride = a.row_stride;
     prm.row_offsets = a.row_offsets;
@@ -828,9 +830,10 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     if (count <= 0) return hipSuccess;
     if (cell_scratch_bytes(a.waves, a.waves1, a.big_cap) > a.scratch_bytes) return hipErrorInvalidValue;
     const int64_t blocks_needed = (count + kGCellsPerBlock - 1) / kGCellsPerBlock;
-    const int64_t blocks = a.waves / 4 < blocks_needed ? a.waves / 4 : blocks_needed;
+    const int64_t max_blocks = (int64_t)a.waves * kWave / kGThreads;
+    const int64_t blocks = max_blocks < blocks_needed ? max_blocks : blocks_needed;
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
-    hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, stream, prm);
+    hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kGThreads), 0, stream, prm);
     hipLaunchKernelGGL(cell_big_kernel, dim3((unsigned)(a.waves1 / 4)), dim3(256), 0, stream, prm);
     return hipGetLastError();
 }

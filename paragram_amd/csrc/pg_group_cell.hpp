@@ -2,14 +2,14 @@
 // together, with the whole cell in LDS.  Included by pg_cell.hip (it uses
 // CellParams, the tier constants and the FP64 helpers defined there).
 //
-// Per cell (3,200 bytes of LDS; 16 cells per 256-thread workgroup):
+// Per cell (2,688 bytes of LDS, plus a 1 KB wait list per wave):
 //   vpos[64]  float4  vertex position y = N/D in seed-relative doubled
 //                     coordinates, and a rigorous error radius (w)
 //   vtri[64]  uint32  plane triple (i, j, k), ordered so that D > 0
 //   pl[48]    float4  plane normal a and offset H (a . y <= H)
 //   plx[48]   float2  permanent of H and the owner (sorted point id, or a
 //                     negative box-plane code)
-//   lref/lkey[64]     traversal wait list, tmp[64] emission scratch
+//   tmp[64]   uint32  clip and emission scratch
 // Vertices live in a pool of 64 slots with a group-uniform valid mask, so a
 // clip frees and refills slots without moving data.  A cell that outgrows
 // the pool (64 vertices, 48 planes or 48 neighbours) is queued for tier 1.
@@ -23,11 +23,16 @@
 
 namespace pg {
 
-constexpr int kGL = 16;                  // lanes per cell
+#ifndef PG_GROUP_LANES
+#define PG_GROUP_LANES 16
+#endif
+constexpr int kGL = PG_GROUP_LANES;      // lanes per cell
+constexpr uint32_t kGMask = (1u << kGL) - 1u;
 constexpr int kGV = 64;                  // vertex slots (valid mask is 64 bits)
 constexpr int kGP = 48;                  // plane slots
 constexpr int kGStack = 64;
-constexpr int kGCellsPerBlock = 16;      // 256 threads
+constexpr int kGThreads = 256;           // threads per workgroup
+constexpr int kGCellsPerBlock = kGThreads / kGL;
 static_assert(kGP <= 64 && kGP <= kRowCap, "plane masks are 64 bits; rows fit the tier-0 stride");
 
 struct alignas(16) GroupLDS {
@@ -35,9 +40,14 @@ struct alignas(16) GroupLDS {
     float4 pl[kGP];
     float2 plx[kGP];
     uint32_t vtri[kGV];
-    uint32_t lref[kGStack];   // wait list: parent << 1 | child
-    float lkey[kGStack];      // wait list keys
     uint32_t tmp[kGV];
+};
+
+// Best-first wait list shared by the cells of one wave (packet traversal).
+constexpr int kWList = 128;
+struct alignas(16) WaveList {
+    uint32_t ref[kWList];     // parent << 1 | child
+    float key[kWList];
 };
 
 struct Cand {
@@ -66,20 +76,25 @@ struct Group {
     float sx, sy, sz, sw;
     uint64_t valid;
     int np;
-    float L[3], U[3], R8[8];
+    float L[3], U[3], L2[3], U2[3];  // AABB of the cell and its squared ends
     bool bounded;
     int status;
     bool overflow;
+    bool active;    // the group holds a cell in this round
+    bool dirty;     // the cell changed since the AABB was computed
     uint64_t bloom;
     bool in_guess;
+    WaveList *W;
+    int lane64;
 
     uint32_t c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands, c_nodes, c_leaves, c_skipped;
 
     // ---------------------------------------------------------- group ops
     __device__ __forceinline__ uint32_t gballot(bool p) const {
-        return (uint32_t)(__ballot(p) >> gshift) & 0xFFFFu;
+        return (uint32_t)(__ballot(p) >> gshift) & kGMask;
     }
     __device__ __forceinline__ bool gany(bool p) const { return gballot(p) != 0u; }
+    __device__ __forceinline__ bool live() const { return active && status == kSuccess && !overflow; }
     __device__ __forceinline__ float gbf(float x, int k) const { return __shfl(x, k, kGL); }
     __device__ __forceinline__ int gbi(int x, int k) const { return __shfl(x, k, kGL); }
     __device__ __forceinline__ uint64_t gor64(uint64_t x) const {
@@ -285,11 +300,7 @@ struct Group {
         bounded = !gany(!ok);
         L[0] = gmin(lo0), L[1] = gmin(lo1), L[2] = gmin(lo2);
         U[0] = gmax(hi0), U[1] = gmax(hi1), U[2] = gmax(hi2);
-        const float l0 = L[0] * L[0], l1 = L[1] * L[1], l2 = L[2] * L[2];
-        const float u0 = U[0] * U[0], u1 = U[1] * U[1], u2 = U[2] * U[2];
-#pragma unroll
-        for (int o = 0; o < 8; ++o)
-            R8[o] = ((o & 1 ? u0 : l0) + (o & 2 ? u1 : l1) + (o & 4 ? u2 : l2)) * (1.0f + kSlack);
+        for (int k = 0; k < 3; ++k) L2[k] = L[k] * L[k], U2[k] = U[k] * U[k];
     }
 
     __device__ __forceinline__ bool too_far(const Cand &c) const {
@@ -314,15 +325,12 @@ struct Group {
         const float hlow = d2 - dw;
         key = hlow;
         if (!bounded) return false;
-        const bool nx_ = alx <= 0.0f, px_ = ahx >= 0.0f;
-        const bool ny_ = aly <= 0.0f, py_ = ahy >= 0.0f;
-        const bool nz_ = alz <= 0.0f, pz_ = ahz >= 0.0f;
-        float rho2 = 0.0f;
-#pragma unroll
-        for (int o = 0; o < 8; ++o) {
-            const bool in = ((o & 1) ? px_ : nx_) && ((o & 2) ? py_ : ny_) && ((o & 4) ? pz_ : nz_);
-            rho2 = in ? fmaxf(rho2, R8[o]) : rho2;
-        }
+        // the octants a box overlaps form a product set, so the largest
+        // corner radius over them is a sum of per-axis maxima
+        const float rho2 = (fmaxf(alx <= 0.0f ? L2[0] : 0.0f, ahx >= 0.0f ? U2[0] : 0.0f) +
+                            fmaxf(aly <= 0.0f ? L2[1] : 0.0f, ahy >= 0.0f ? U2[1] : 0.0f) +
+                            fmaxf(alz <= 0.0f ? L2[2] : 0.0f, ahz >= 0.0f ? U2[2] : 0.0f)) *
+                           (1.0f + kSlack);
         const float d2lo = d2 * (1.0f - kSlack);
         const float h = d2lo - dw - kSlack * (fabsf(lo.w) + fabsf(sw)) - kTiny32;
         if (4.0f * d2lo > rho2 && h > 0.0f && h * h > rho2 * d2 * (1.0f + kSlack)) return true;
@@ -403,7 +411,7 @@ struct Group {
         uint64_t rem = 0;
 #pragma unroll
         for (int k = 0; k < kGV / kGL; ++k) {
-            if (((valid >> (kGL * k)) & 0xFFFFull) == 0ull) continue;
+            if (((valid >> (kGL * k)) & (uint64_t)kGMask) == 0ull) continue;
             const int v = gl + kGL * k;
             const bool r = ((valid >> v) & 1ull) && beyond(v, q);
             rem |= (uint64_t)gballot(r) << (kGL * k);
@@ -426,8 +434,64 @@ struct Group {
             S->plx[qs] = make_float2(q.pH, __int_as_float(q.owner));
         }
         if (in_guess) bloom |= bloom_bit((uint32_t)q.owner);
-        // Boundary of the removed triangles: directed edges whose twin is
-        // not removed (each lane: the removed slots v = gl + 16 k).
+        // Boundary of the removed triangles: directed edges whose twin is not
+        // removed.  The R removed triangles are spread one per lane (through
+        // tmp[]), and each lane checks its three edges against all of them.
+        if (R > kGL) return clip_wide(q, rem, qs);
+        int base = 0;
+#pragma unroll
+        for (int k = 0; k < kGV / kGL; ++k) {
+            const uint32_t bits = (uint32_t)(rem >> (kGL * k)) & kGMask;
+            if ((bits >> gl) & 1u) S->tmp[base + __popc(bits & ((1u << gl) - 1u))] = S->vtri[gl + kGL * k];
+            base += __popc(bits);
+        }
+        const bool mine_t = gl < R;
+        const uint32_t tr = mine_t ? S->tmp[gl] : 0u;
+        uint32_t edge[3];
+        uint32_t emask = 0;
+        int sx_ = 0, sy_ = 0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const uint32_t x = tri_get<8>(tr, e), y = tri_get<8>(tr, (e + 1) % 3);
+            bool twin = false;
+            for (int u = 0; u < R; ++u) twin |= tri_has_edge<8>((uint32_t)__shfl((int)tr, u, kGL), y, x);
+            edge[e] = x | (y << 8);
+            if (mine_t && !twin) {
+                emask |= 1u << e;
+                sx_ += (int)(x * x + 1u);
+                sy_ += (int)(y * y + 1u);
+            }
+        }
+        int nb;
+        int j = gexcl(__popc(emask), nb);
+        // every plane of the boundary cycle starts one edge and ends one
+        if (nb < 3 || gsum(sx_) != gsum(sy_)) {
+            status = kInconsistentBoundary;
+            if (gl == 0) c_inconsistent++;
+            return true;
+        }
+        const uint64_t freem = ~(valid & ~rem);
+        if (nb > __popcll(freem)) {
+            overflow = true;
+            return false;
+        }
+        uint64_t newbits = 0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (!((emask >> e) & 1u)) continue;
+            const int slot = select_bit(freem, j++);
+            make_vertex(slot, (uint32_t)qs, edge[e] & 255u, edge[e] >> 8);
+            newbits |= 1ull << slot;
+        }
+        valid = (valid & ~rem) | gor64(newbits);
+        dirty = true;
+        return true;
+    }
+
+    // Boundary pass for clips that remove more triangles than a group has
+    // lanes (early clips of the initial box): each lane walks its own rows.
+    __device__ bool clip_wide(const Cand &q, uint64_t rem, int qs) {
+        (void)q;
         uint32_t edge[kGV / kGL * 3];
         uint32_t emask = 0;
         int sx_ = 0, sy_ = 0;
@@ -458,7 +522,6 @@ struct Group {
         }
         int nb;
         int j = gexcl(__popc(emask), nb);
-        // every plane of the boundary cycle starts one edge and ends one
         if (nb < 3 || gsum(sx_) != gsum(sy_)) {
             status = kInconsistentBoundary;
             if (gl == 0) c_inconsistent++;
@@ -478,7 +541,7 @@ struct Group {
             newbits |= 1ull << slot;
         }
         valid = (valid & ~rem) | gor64(newbits);
-        update_bounds();
+        dirty = true;
         return true;
     }
 
@@ -529,12 +592,13 @@ struct Group {
             if (clip(q)) {
                 if (gl == 0) c_clips++;
                 if (status != kSuccess) return;
-                // the cell shrank: drop the batch's candidates it now excludes
-                live = live && !too_far(c);
-                todo &= gballot(live);
             } else if (overflow) {
                 return;
             }
+        }
+        if (dirty) {
+            update_bounds();
+            dirty = false;
         }
     }
 
@@ -562,6 +626,7 @@ struct Group {
         valid = 0xFFull;
         bloom = 0;
         update_bounds();
+        dirty = false;
     }
 
     // ---------------------------------------------------------- traversal
@@ -574,24 +639,24 @@ struct Group {
         }
     }
 
-    // Smallest-key entry of the wait list (group arg-min), -1 when empty.
+    // Smallest-key entry of the wave's wait list (wave arg-min), -1 if empty.
     __device__ int list_argmin(int sz, float &kmin) const {
         float k = FLT_MAX;
         int idx = -1;
 #pragma unroll
-        for (int j = 0; j < kGStack / kGL; ++j) {
-            const int t = gl + kGL * j;
+        for (int j = 0; j < kWList / kWave; ++j) {
+            const int t = lane64 + kWave * j;
             if (t < sz) {
-                const float kt = S->lkey[t];
-                if (kt < k || idx < 0) {
+                const float kt = W->key[t];
+                if (idx < 0 || kt < k) {
                     k = kt;
                     idx = t;
                 }
             }
         }
-        for (int o = kGL / 2; o > 0; o >>= 1) {
-            const float ok = __shfl_xor(k, o, kGL);
-            const int oi = __shfl_xor(idx, o, kGL);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float ok = __shfl_xor(k, o);
+            const int oi = __shfl_xor(idx, o);
             if (oi >= 0 && (idx < 0 || ok < k || (ok == k && oi < idx))) {
                 k = ok;
                 idx = oi;
@@ -601,38 +666,52 @@ struct Group {
         return idx;
     }
 
-    // Best-first traversal (the order voronoi_bvh.cuh:246-283 keeps with its
-    // min-extraction pops): unpruned children wait in a keyed list in LDS
-    // (key = lower bound of H over the child's box); the group always goes
-    // on with the smallest key, re-testing a waiting entry against the cell
-    // as it is when the entry comes out.  A full list sends the cell to the
-    // large-capacity tier instead of dropping entries.
+    // Wave-wide: does any group still need this box?  key: smallest bound.
+    __device__ bool packet_needs(float4 lo, float4 hi, float &key) const {
+        float k = FLT_MAX;
+        const bool need = live() && !node_prunable(lo, hi, k);
+        const bool any = __ballot(gl == 0 && need) != 0ull;
+        float kk = need ? k : FLT_MAX;
+        for (int o = kGL; o < kWave; o <<= 1) kk = fminf(kk, __shfl_xor(kk, o));
+        key = kk;
+        return any;
+    }
+
+    // Best-first packet traversal: the four cells of a wave (Morton
+    // neighbours) walk the hierarchy together.  A box is expanded while any
+    // of them may still be cut by a point in it; its key is the smallest
+    // lower bound of H over the cells that need it; leaves are offered to
+    // every cell, each filtering and clipping on its own lanes.  Unpruned
+    // children wait in one wave-wide keyed list and the wave always goes on
+    // with the smallest key (the order voronoi_bvh.cuh:246-283 keeps with its
+    // min-extraction pops), re-testing a waiting box when it comes out.  A
+    // full list sends the wave's cells to the large-capacity tier instead of
+    // dropping entries.
     __device__ void traverse() {
         uint32_t ref = root_ref;
         int sz = 0;
         while (true) {
             bool have = false;
             if (is_leaf(ref)) {
-                offer_leaf(ref);
-                if (status != kSuccess || overflow) return;
+                if (live()) offer_leaf(ref);
             } else {
-                if (gl == 0) c_nodes++;
+                if (lane64 == 0) c_nodes++;
                 const Node nd = nodes[ref];
                 float k0, k1;
-                const bool p0 = node_prunable(nd.lo0, nd.hi0, k0);
-                const bool p1 = node_prunable(nd.lo1, nd.hi1, k1);
+                const bool n0 = packet_needs(nd.lo0, nd.hi0, k0);
+                const bool n1 = packet_needs(nd.lo1, nd.hi1, k1);
                 const uint32_t r0 = __float_as_uint(nd.hi0.w), r1 = __float_as_uint(nd.hi1.w);
-                if (!(p0 && p1)) {
-                    const bool near0 = p1 || (!p0 && k0 <= k1);
+                if (n0 || n1) {
+                    const bool near0 = !n1 || (n0 && k0 <= k1);
                     const float knear = near0 ? k0 : k1;
-                    if (!p0 && !p1) {  // the farther child waits
-                        if (sz >= kGStack) {
+                    if (n0 && n1) {  // the farther child waits
+                        if (sz >= kWList) {
                             overflow = true;
                             return;
                         }
-                        if (gl == 0) {
-                            S->lref[sz] = (ref << 1) | (near0 ? 1u : 0u);
-                            S->lkey[sz] = near0 ? k1 : k0;
+                        if (lane64 == 0) {
+                            W->ref[sz] = (ref << 1) | (near0 ? 1u : 0u);
+                            W->key[sz] = near0 ? k1 : k0;
                         }
                         sz++;
                     }
@@ -641,14 +720,14 @@ struct Group {
                     if (im < 0 || knear <= kmin) {
                         ref = near0 ? r0 : r1;
                         have = true;
-                    } else {  // a waiting entry is nearer: the near child waits too
-                        if (sz >= kGStack) {
+                    } else {  // a waiting box is nearer: the near child waits too
+                        if (sz >= kWList) {
                             overflow = true;
                             return;
                         }
-                        if (gl == 0) {
-                            S->lref[sz] = (ref << 1) | (near0 ? 0u : 1u);
-                            S->lkey[sz] = knear;
+                        if (lane64 == 0) {
+                            W->ref[sz] = (ref << 1) | (near0 ? 0u : 1u);
+                            W->key[sz] = knear;
                         }
                         sz++;
                     }
@@ -658,10 +737,10 @@ struct Group {
                 float kmin;
                 const int im = list_argmin(sz, kmin);
                 if (im < 0) return;
-                const uint32_t e = S->lref[im];
-                if (gl == 0) {
-                    S->lref[im] = S->lref[sz - 1];
-                    S->lkey[im] = S->lkey[sz - 1];
+                const uint32_t e = W->ref[im];
+                if (lane64 == 0) {
+                    W->ref[im] = W->ref[sz - 1];
+                    W->key[im] = W->key[sz - 1];
                 }
                 sz--;
                 const uint32_t parent = e >> 1, c = e & 1u;
@@ -669,7 +748,7 @@ struct Group {
                 float k;
                 const float4 lo = c ? nd.lo1 : nd.lo0;
                 const float4 hi = c ? nd.hi1 : nd.hi0;
-                if (!node_prunable(lo, hi, k)) {
+                if (packet_needs(lo, hi, k)) {
                     ref = __float_as_uint(hi.w);
                     have = true;
                 }
@@ -748,7 +827,7 @@ struct Group {
         return true;
     }
 
-    __device__ void run(uint32_t cell, const CellParams &prm) {
+    __device__ void start(uint32_t cell, const CellParams &prm) {
         seed = cell;
         const float4 s = pts[cell];
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
@@ -757,6 +836,13 @@ struct Group {
         init_box();
         const uint32_t orig = perm[cell];
         in_guess = true;
+        if (!prm.guesses && prm.seed_window > 0) {
+            // Morton neighbours first: the cheap analogue of clipping by the
+            // k nearest neighbours (voronoi_ultra.cu:70-85); order does not
+            // change the result, only how fast the cell shrinks
+            const int64_t j = (int64_t)cell - kGL / 2 + gl + (gl >= kGL / 2 ? 1 : 0);
+            offer_batch(gl < prm.seed_window && j >= 0 && j < prm.n, (uint32_t)(j < 0 ? 0 : j));
+        }
         if (prm.guesses) {
             const int32_t g0 = prm.guess_offsets[orig], g1 = prm.guess_offsets[orig + 1];
             for (int32_t g = g0; g < g1 && status == kSuccess && !overflow; g += kGL) {
@@ -772,7 +858,10 @@ struct Group {
             }
         }
         in_guess = false;
-        if (status == kSuccess && !overflow) traverse();
+    }
+
+    __device__ void finish(uint32_t cell, const CellParams &prm) {
+        const uint32_t orig = perm[cell];
         const int64_t local = (int64_t)cell - prm.begin;
         if (!overflow) {
             if (gl == 0) prm.row_offsets[cell] = local * prm.row_stride;
@@ -793,15 +882,19 @@ struct Group {
     }
 };
 
-// Capacity tier 0: 16 lanes per cell, cells pulled one at a time per group
-// from a global counter, state in LDS.
-__global__ void __launch_bounds__(256) cell_group_kernel(const CellParams prm) {
+// Capacity tier 0: 16 lanes per cell, a wave holds four Morton-consecutive
+// cells per round (pulled from a global counter) and traverses for them
+// together; cell state in LDS.
+__global__ void __launch_bounds__(kGThreads) cell_group_kernel(const CellParams prm) {
     __shared__ GroupLDS lds[kGCellsPerBlock];
+    __shared__ WaveList wlist[kGThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int grp = lane / kGL;
     Group g;
     g.S = &lds[(threadIdx.x / kWave) * (kWave / kGL) + grp];
+    g.W = &wlist[threadIdx.x / kWave];
     g.gl = lane & (kGL - 1);
+    g.lane64 = lane;
     g.gshift = grp * kGL;
     g.pts = prm.pts;
     g.perm = prm.perm;
@@ -813,15 +906,24 @@ __global__ void __launch_bounds__(256) cell_group_kernel(const CellParams prm) {
     g.c_nodes = g.c_leaves = g.c_skipped = 0;
     uint32_t cells = 0;
     const int64_t count = prm.end - prm.begin;
+    constexpr int kPacket = kWave / kGL;
     while (true) {
-        uint32_t c = 0;
-        if (g.gl == 0) c = atomicAdd(&prm.counters[0], 1u);
-        c = (uint32_t)__shfl((int)c, 0, kGL);
-        if ((int64_t)c >= count) break;
+        uint32_t c0 = 0;
+        if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kPacket);
+        c0 = (uint32_t)__shfl((int)c0, 0);
+        if ((int64_t)c0 >= count) break;
         const uint64_t t0 = clock64();
-        g.run((uint32_t)(prm.begin + c), prm);
-        if (prm.cycles && g.gl == 0) prm.cycles[prm.perm[prm.begin + c]] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
-        if (g.gl == 0) cells++;
+        const int64_t c = (int64_t)c0 + grp;
+        g.active = c < count;
+        const uint32_t cell = (uint32_t)(prm.begin + (g.active ? c : 0));
+        if (g.active) g.start(cell, prm);
+        g.traverse();
+        if (g.active) {
+            g.finish(cell, prm);
+            if (prm.cycles && g.gl == 0)
+                prm.cycles[prm.perm[cell]] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+            if (g.gl == 0) cells++;
+        }
     }
     const uint32_t vals[12] = {g.c_recheck, g.c_unresolved, g.c_inconsistent, g.c_overflow, g.c_clips, g.c_cands,
                                g.c_nodes, cells, 0, 0, g.c_leaves, g.c_skipped};

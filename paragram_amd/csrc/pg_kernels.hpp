@@ -21,6 +21,7 @@ struct CellArgs {
     const int32_t *guesses;       // may be null
     const int32_t *guess_offsets; // indexed by original id (null with guesses == null)
     int guesses_sorted;           // guesses already hold sorted indices (kNN output)
+    int seed_window;              // Morton neighbours offered before the traversal
     int32_t *rows;                // (end - begin) x row_stride, then big_cap x kTier1Row
     int row_stride;
     int64_t *row_offsets;         // [n] sorted cell -> row start in rows

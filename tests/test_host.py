@@ -174,6 +174,6 @@ def test_library_argument_errors_without_gpu():
     assert b"non-negative" in lib.lib.pg_last_error()
     assert lib.lib.pg_index_workspace_bytes(1 << 27, ctypes.byref(out)) == 1
     assert lib.lib.pg_index_workspace_bytes(1000, ctypes.byref(out)) == 0 and out.value > 1000 * 16
-    assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, None, None, None, None, None, None, 0, 1,
+    assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, 16, None, None, None, None, None, None, 0, 1,
                                     None, None) == 1
     assert b"out of bounds" in lib.lib.pg_last_error()
