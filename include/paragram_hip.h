@@ -72,8 +72,8 @@ int pg_build_index(int64_t n, const float *points, const float *weights, int32_t
 int pg_default_waves(int32_t *waves);
 
 /* Bytes of cell scratch for `waves` persistent waves and launches over at
- * most max_cells cells (per-wave state of both capacity tiers plus the
- * tier-1 queue).
+ * most max_cells cells (work counters and the queues of the two
+ * large-capacity tiers; the cells themselves live in LDS).
  * Replaces ConvexCellMemory::required_bytes (voronoi_convex_cell.cuh:37-44). */
 int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes);
 
@@ -91,7 +91,7 @@ int pg_row_buffer_elems(int64_t cells, uint64_t *elems);
  * changes speed only, never the result).
  * stats: PG_NUM_STATS counters, accumulated.
  * cell_cycles: optional (may be null) per-cell cost in GPU clock cycles,
- * original order; bit 31 set marks cells built by the large-capacity tier.
+ * original order; bit 31 set marks cells built by a large-capacity tier.
  * Replaces the chunk loop and compute_bvh_{voronoi,power_diagram}_kernel
  * (voronoi_ultra.cu:39-141,283-305; laguerre_ultra.cu:43-174,337-357). */
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,

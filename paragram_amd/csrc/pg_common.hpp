@@ -7,28 +7,28 @@
 namespace pg {
 
 // ------------------------------------------------------------------ limits
-// Plane and vertex capacity of one cell, in two tiers.  A cell of F faces has
-// 2F-4 vertices; unreferenced planes are compacted when the plane list fills
-// up.  Tier 0 holds nearly every cell; the few that outgrow it (large sparse
-// cells, e.g. near-range LiDAR returns with 200+ faces) are re-run in tier 1.
-constexpr int kTier0Planes = 96;
-constexpr int kTier0Verts = 192;
-constexpr int kTier0Bits = 8;
-constexpr int kTier1Planes = 512;
-constexpr int kTier1Verts = 1024;
-constexpr int kTier1Bits = 10;
-constexpr int kTier1Row = kTier1Planes;
-constexpr int kRowCap = 48;           // tier-0 row stride (= the tier-0 plane capacity)
+// Row strides of the three capacity tiers of the cell kernel (the tiers and
+// their plane / vertex capacities are defined in pg_group_cell.hpp).  A cell
+// of F faces has 2F-4 vertices.
+constexpr int kRowCap = 48;            // tier-0 row stride (= the tier-0 plane capacity)
+constexpr int kTier1Row = 512;         // tier-1 row stride (255 planes, plus duplicates)
 constexpr int kTier1Waves = 256;       // persistent waves of the tier-1 launch
+constexpr int kTier2Row = 2048;        // tier-2 row stride (1023 planes, plus duplicates)
+constexpr int kTier2Waves = 256;       // persistent waves of the tier-2 launch
 
 // Tier-1 queue capacity for a launch over m cells.
 __host__ __device__ inline int64_t tier1_capacity(int64_t m) {
     const int64_t c = m / 32 + 256;
     return c < m ? c : m;
 }
+// Tier-2 queue capacity for a launch over m cells.
+__host__ __device__ inline int64_t tier2_capacity(int64_t m) {
+    const int64_t c = m / 1024 + 64;
+    return c < m ? c : m;
+}
 // Elements of the row buffer of a launch over m cells.
 __host__ __device__ inline int64_t row_buffer_elems(int64_t m) {
-    return m * kRowCap + tier1_capacity(m) * kTier1Planes;
+    return m * kRowCap + tier1_capacity(m) * kTier1Row + tier2_capacity(m) * kTier2Row;
 }
 constexpr int kStackCap = 64;          // LBVH depth is < 56 for N < 2^26
 constexpr int kMaxLeaf = 32;
@@ -160,40 +160,6 @@ inline IndexLayout make_index_layout(int64_t n) {
     L.dup_head = take(4 * nn);
     L.dup_next = take(4 * nn);
     L.total = o;
-    return L;
-}
-
-// ------------------------------------------------------------------ cell scratch
-// Per-lane cell state lives in a scratch buffer owned by the caller, laid out
-// per wave as [slot][lane] so that a wave touching the same slot in every
-// lane reads 64 consecutive elements.
-struct CellScratchLayout {
-    uint64_t plane_ah;    // float4 [kPlaneCap]  (a.x, a.y, a.z, H)
-    uint64_t plane_aux;   // float2 [kPlaneCap]  (permanent of H, owner bits)
-    uint64_t vert_tri;    // uint32 [kVertCap]   packed plane triple
-    uint64_t vert_nd;     // float4 [kVertCap]   homogeneous position (N, D)
-    uint64_t vert_perm;   // float2 [kVertCap]   permanents (max N, D)
-    uint64_t next;        // uint32 [kPlaneCap]  boundary successor, generation tagged
-    uint64_t stack;       // uint32 [kStackCap]  traversal stack
-    uint64_t per_wave;    // bytes per wave
-};
-
-__host__ __device__ inline CellScratchLayout make_cell_scratch_layout(int kPlaneCap, int kVertCap) {
-    CellScratchLayout L{};
-    uint64_t o = 0;
-    auto take = [&](uint64_t bytes) {
-        uint64_t at = o;
-        o = align_up(o + bytes * kWave, 256);
-        return at;
-    };
-    L.plane_ah = take(16ull * kPlaneCap);
-    L.plane_aux = take(8ull * kPlaneCap);
-    L.vert_tri = take(4ull * kVertCap);
-    L.vert_nd = take(16ull * kVertCap);
-    L.vert_perm = take(8ull * kVertCap);
-    L.next = take(4ull * kPlaneCap);
-    L.stack = take(4ull * kStackCap);
-    L.per_wave = o;
     return L;
 }
 

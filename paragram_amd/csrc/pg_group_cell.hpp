@@ -1,53 +1,52 @@
-// Capacity tier 0 of the cell construction: 16 lanes build one cell
-// together, with the whole cell in LDS.  Included by pg_cell.hip (it uses
-// CellParams, the tier constants and the FP64 helpers defined there).
+// Cooperative cell construction: G lanes build one cell together, with the
+// whole cell in LDS.  Included by pg_cell.hip (it uses CellParams, the
+// precision constants and the plane-triple helpers defined there).
 //
-// Per cell (2,688 bytes of LDS, plus a 1 KB wait list per wave):
-//   vpos[64]  float4  vertex position y = N/D in seed-relative doubled
-//                     coordinates, and a rigorous error radius (w)
-//   vtri[64]  uint32  plane triple (i, j, k), ordered so that D > 0
-//   pl[48]    float4  plane normal a and offset H (a . y <= H)
-//   plx[48]   float2  permanent of H and the owner (sorted point id, or a
-//                     negative box-plane code)
-//   tmp[64]   uint32  clip and emission scratch
-// Vertices live in a pool of 64 slots with a group-uniform valid mask, so a
-// clip frees and refills slots without moving data.  A cell that outgrows
-// the pool (64 vertices, 48 planes or 48 neighbours) is queued for tier 1.
+// Three instances (capacity tiers):
+//   tier 0: G = 16 lanes, 64 vertex slots, 48 plane slots (2,688 B of LDS per
+//           cell, 4 cells per wave) -- every cell starts here;
+//   tier 1: G = 64 lanes, 512 vertex slots, 255 plane slots (one cell per
+//           wave) -- the few cells that outgrow tier 0 (cells with more than
+//           about 34 faces, e.g. near-range LiDAR returns with 200+ faces);
+//   tier 2: G = 64 lanes, 1024 vertex slots, 1023 plane slots, 10-bit plane
+//           indices (one cell per wave and block) -- the rare cells that
+//           outgrow tier 1 (or its 256-entry traversal list).
 //
-// Lane parallelism: vertex sweeps (slot v on lane v mod 16), the batch of up
-// to 16 candidates of a leaf (one per lane, then offered in order), the
-// boundary edges of a clip and the new vertices they make, bounds
-// reductions and the row sort.  All control decisions are group-uniform
-// (ballots restricted to the group's 16 lanes).
+// Per cell in LDS:
+//   vpos[V]  float4  vertex position y = N/D in seed-relative doubled
+//                    coordinates, and a rigorous error radius (w)
+//   vtri[V]  uint32  plane triple (i, j, k), ordered so that D > 0
+//   pl[P]    float4  plane normal a and offset H (a . y <= H)
+//   plx[P]   float2  permanent of H and the owner (sorted point id, or a
+//                    negative box-plane code)
+//   tmp[V]   uint32  clip and emission scratch
+// Vertices live in a pool of V slots with a group-uniform valid mask (slot
+// v = lane + G k belongs to lane v mod G), so a clip frees and refills slots
+// without moving data.
+//
+// Lane parallelism: vertex sweeps, the batch of up to G candidates of a leaf
+// (one per lane, then offered in order), the boundary edges of a clip and
+// the new vertices they make, bounds reductions and the row sort.  All
+// control decisions are group-uniform (ballots restricted to the group).
+// The cells of one wave (Morton neighbours) share one best-first traversal.
 #pragma once
 
 namespace pg {
 
-#ifndef PG_GROUP_LANES
-#define PG_GROUP_LANES 16
-#endif
-constexpr int kGL = PG_GROUP_LANES;      // lanes per cell
-constexpr uint32_t kGMask = (1u << kGL) - 1u;
-constexpr int kGV = 64;                  // vertex slots (valid mask is 64 bits)
-constexpr int kGP = 48;                  // plane slots
-constexpr int kGStack = 64;
-constexpr int kGThreads = 256;           // threads per workgroup
-constexpr int kGCellsPerBlock = kGThreads / kGL;
-static_assert(kGP <= 64 && kGP <= kRowCap, "plane masks are 64 bits; rows fit the tier-0 stride");
-
-struct alignas(16) GroupLDS {
-    float4 vpos[kGV];
-    float4 pl[kGP];
-    float2 plx[kGP];
-    uint32_t vtri[kGV];
-    uint32_t tmp[kGV];
+// Best-first wait list shared by the cells of one wave (packet traversal).
+template <int L>
+struct alignas(16) WaveList {
+    uint32_t ref[L];  // parent << 1 | child
+    float key[L];
 };
 
-// Best-first wait list shared by the cells of one wave (packet traversal).
-constexpr int kWList = 128;
-struct alignas(16) WaveList {
-    uint32_t ref[kWList];     // parent << 1 | child
-    float key[kWList];
+template <int V, int P>
+struct alignas(16) GroupLDS {
+    float4 vpos[V];
+    float4 pl[P];
+    float2 plx[P];
+    uint32_t vtri[V];
+    uint32_t tmp[V];
 };
 
 struct Cand {
@@ -55,15 +54,72 @@ struct Cand {
     int owner;
 };
 
-__device__ __forceinline__ int select_bit(uint64_t m, int j) {
-    for (int t = 0; t < j; ++t) m &= m - 1;
-    return (int)__builtin_ctzll(m);
+// ------------------------------------------------------------------ masks
+// A mask of W 64-bit words; slot v is bit v & 63 of word v >> 6.
+template <int W>
+__device__ __forceinline__ int mask_popc(const uint64_t (&m)[W]) {
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c += __popcll(m[w]);
+    return c;
+}
+template <int W>
+__device__ __forceinline__ bool mask_bit(const uint64_t (&m)[W], int v) {
+    uint64_t word = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) word = (v >> 6) == w ? m[w] : word;
+    return (word >> (v & 63)) & 1ull;
+}
+// index of the j-th set bit (j < popcount)
+template <int W>
+__device__ __forceinline__ int mask_select(const uint64_t (&m)[W], int j) {
+    int base = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int c = __popcll(m[w]);
+        if (j < c) {
+            uint64_t x = m[w];
+            for (int t = 0; t < j; ++t) x &= x - 1ull;
+            return base + (int)__builtin_ctzll(x);
+        }
+        j -= c;
+        base += 64;
+    }
+    return -1;
+}
+// the lowest n set bits of m
+template <int W>
+__device__ __forceinline__ void mask_first(const uint64_t (&m)[W], int n, uint64_t (&out)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t x = m[w], keep = 0;
+        while (n > 0 && x) {
+            const uint64_t low = x & (~x + 1ull);
+            keep |= low;
+            x ^= low;
+            n--;
+        }
+        out[w] = keep;
+    }
 }
 
+// ------------------------------------------------------------------ group
+template <int G, int V, int P, int L>
 struct Group {
-    GroupLDS *S;
-    int gl;       // lane within the group
-    int gshift;   // bit offset of the group in a wave ballot
+    static constexpr int kRows = V / G;               // vertex slots per lane
+    static constexpr int kW = V / 64;                 // words of a vertex mask
+    static constexpr int kPW = (P + 63) / 64;         // words of a plane mask
+    static constexpr int kPRows = (P + G - 1) / G;    // plane slots per lane
+    static constexpr uint64_t kRowMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    static constexpr int kB = P < 256 ? 8 : 10;       // bits of a plane slot in a triple
+    static constexpr uint32_t kBMask = (1u << kB) - 1u;
+    static_assert(64 % G == 0 && V % 64 == 0 && P < (1 << kB) && 3 * kRows <= 64, "group geometry");
+
+    GroupLDS<V, P> *S;
+    WaveList<L> *W;
+    int gl;      // lane within the group
+    int gshift;  // bit offset of the group in a wave ballot
+    int lane64;
 
     const float4 *pts;
     const uint32_t *perm;
@@ -74,58 +130,61 @@ struct Group {
 
     uint32_t seed;
     float sx, sy, sz, sw;
-    uint64_t valid;
+    uint64_t valid[kW];
     int np;
-    float L[3], U[3], L2[3], U2[3];  // AABB of the cell and its squared ends
+    float L_[3], U_[3], L2[3], U2[3];  // AABB of the cell and its squared ends
     bool bounded;
     int status;
     bool overflow;
-    bool active;    // the group holds a cell in this round
-    bool dirty;     // the cell changed since the AABB was computed
+    bool active;  // the group holds a cell in this round
+    bool dirty;   // the cell changed since the AABB was computed
     uint64_t bloom;
     bool in_guess;
-    WaveList *W;
-    int lane64;
 
     uint32_t c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands, c_nodes, c_leaves, c_skipped;
 
     // ---------------------------------------------------------- group ops
-    __device__ __forceinline__ uint32_t gballot(bool p) const {
-        return (uint32_t)(__ballot(p) >> gshift) & kGMask;
-    }
-    __device__ __forceinline__ bool gany(bool p) const { return gballot(p) != 0u; }
+    __device__ __forceinline__ uint64_t gballot(bool p) const { return (__ballot(p) >> gshift) & kRowMask; }
+    __device__ __forceinline__ bool gany(bool p) const { return gballot(p) != 0ull; }
     __device__ __forceinline__ bool live() const { return active && status == kSuccess && !overflow; }
-    __device__ __forceinline__ float gbf(float x, int k) const { return __shfl(x, k, kGL); }
-    __device__ __forceinline__ int gbi(int x, int k) const { return __shfl(x, k, kGL); }
+    __device__ __forceinline__ float gbf(float x, int k) const { return __shfl(x, k, G); }
+    __device__ __forceinline__ int gbi(int x, int k) const { return __shfl(x, k, G); }
     __device__ __forceinline__ uint64_t gor64(uint64_t x) const {
         uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-        for (int o = kGL / 2; o > 0; o >>= 1) {
-            lo |= (uint32_t)__shfl_xor((int)lo, o, kGL);
-            hi |= (uint32_t)__shfl_xor((int)hi, o, kGL);
+        for (int o = G / 2; o > 0; o >>= 1) {
+            lo |= (uint32_t)__shfl_xor((int)lo, o, G);
+            hi |= (uint32_t)__shfl_xor((int)hi, o, G);
         }
         return ((uint64_t)hi << 32) | lo;
     }
     __device__ __forceinline__ float gmin(float x) const {
-        for (int o = kGL / 2; o > 0; o >>= 1) x = fminf(x, __shfl_xor(x, o, kGL));
+        for (int o = G / 2; o > 0; o >>= 1) x = fminf(x, __shfl_xor(x, o, G));
         return x;
     }
     __device__ __forceinline__ float gmax(float x) const {
-        for (int o = kGL / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, kGL));
+        for (int o = G / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, G));
         return x;
     }
     __device__ __forceinline__ int gsum(int x) const {
-        for (int o = kGL / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kGL);
+        for (int o = G / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, G);
         return x;
     }
     // exclusive prefix sum over the group; total in `tot`
     __device__ __forceinline__ int gexcl(int x, int &tot) const {
         int s = x;
-        for (int o = 1; o < kGL; o <<= 1) {
-            const int y = __shfl_up(s, o, kGL);
+        for (int o = 1; o < G; o <<= 1) {
+            const int y = __shfl_up(s, o, G);
             if (gl >= o) s += y;
         }
-        tot = __shfl(s, kGL - 1, kGL);
+        tot = __shfl(s, G - 1, G);
         return s - x;
+    }
+    // row k of a vertex mask: the bits of slots gl + G k, lane order
+    __device__ __forceinline__ uint64_t row(const uint64_t (&m)[kW], int k) const {
+        return (m[(G * k) >> 6] >> ((G * k) & 63)) & kRowMask;
+    }
+    __device__ __forceinline__ bool own(const uint64_t (&m)[kW], int k) const {
+        return (row(m, k) >> gl) & 1ull;
     }
 
     // ---------------------------------------------------------- FP64 path
@@ -159,7 +218,7 @@ struct Group {
     __device__ void vertex64(uint32_t t, double N[3], double &D, double pN[3], double &pD) const {
         double a[3][3], H[3], pa[3][3], pH[3];
         for (int e = 0; e < 3; ++e) {
-            const int owner = __float_as_int(S->plx[tri_get<8>(t, e)].y);
+            const int owner = __float_as_int(S->plx[tri_get<kB>(t, e)].y);
             plane64(owner, a[e], H[e], pa[e], pH[e]);
         }
         double c[3][3], pc[3][3];
@@ -197,7 +256,7 @@ struct Group {
         // vertex's planes gives the same plane: P = 0 exactly, not beyond.
         const float4 q = pts[q_owner];
         for (int e = 0; e < 3; ++e) {
-            const int o = __float_as_int(S->plx[tri_get<8>(t, e)].y);
+            const int o = __float_as_int(S->plx[tri_get<kB>(t, e)].y);
             if (o < 0) continue;
             const float4 r = pts[o];
             if (r.x == q.x && r.y == q.y && r.z == q.z && r.w == q.w) return false;
@@ -249,7 +308,7 @@ struct Group {
         const float pNx = pHA * q1x + pHB * q2x + pHC * q3x;
         const float pNy = pHA * q1y + pHB * q2y + pHC * q3y;
         const float pNz = pHA * q1z + pHB * q2z + pHC * q3z;
-        const uint32_t tri = tri_pack<8>(i, j, k);
+        const uint32_t tri = tri_pack<kB>(i, j, k);
         const float eD = kG32 * pD;
         float4 pos;
         if (D > 2.0f * eD && pD > kTiny32) {
@@ -273,9 +332,9 @@ struct Group {
     __device__ __forceinline__ bool beyond(int v, const Cand &q) {
         const float4 p = S->vpos[v];
         const float tx = q.ax * p.x, ty = q.ay * p.y, tz = q.az * p.z;
-        const float P = tx + ty + tz - q.H;
+        const float Pv = tx + ty + tz - q.H;
         const float err = q.pa1 * p.w * (1.0f + kSlack) + kG32 * (fabsf(tx) + fabsf(ty) + fabsf(tz) + q.pH) + kTiny32;
-        if (fabsf(P) > err) return P > 0.0f;
+        if (fabsf(Pv) > err) return Pv > 0.0f;
         return recheck64(v, q.owner);
     }
 
@@ -284,10 +343,9 @@ struct Group {
         float lo0 = FLT_MAX, lo1 = FLT_MAX, lo2 = FLT_MAX, hi0 = -FLT_MAX, hi1 = -FLT_MAX, hi2 = -FLT_MAX;
         bool ok = true;
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            const int v = gl + kGL * k;
-            if ((valid >> v) & 1ull) {
-                const float4 p = S->vpos[v];
+        for (int k = 0; k < kRows; ++k) {
+            if (own(valid, k)) {
+                const float4 p = S->vpos[gl + G * k];
                 ok &= p.w < FLT_MAX;
                 lo0 = fminf(lo0, p.x - p.w);
                 hi0 = fmaxf(hi0, p.x + p.w);
@@ -298,22 +356,32 @@ struct Group {
             }
         }
         bounded = !gany(!ok);
-        L[0] = gmin(lo0), L[1] = gmin(lo1), L[2] = gmin(lo2);
-        U[0] = gmax(hi0), U[1] = gmax(hi1), U[2] = gmax(hi2);
-        for (int k = 0; k < 3; ++k) L2[k] = L[k] * L[k], U2[k] = U[k] * U[k];
+        L_[0] = gmin(lo0), L_[1] = gmin(lo1), L_[2] = gmin(lo2);
+        U_[0] = gmax(hi0), U_[1] = gmax(hi1), U_[2] = gmax(hi2);
+        for (int k = 0; k < 3; ++k) L2[k] = L_[k] * L_[k], U2[k] = U_[k] * U_[k];
     }
 
+    // Can the plane of candidate c not cut the cell?  max over the AABB of
+    // a . y is sum_k max(a_k L_k, a_k U_k).
     __device__ __forceinline__ bool too_far(const Cand &c) const {
         if (!bounded) return false;
-        const float S_ = fmaxf(c.ax * L[0], c.ax * U[0]) + fmaxf(c.ay * L[1], c.ay * U[1]) +
-                         fmaxf(c.az * L[2], c.az * U[2]);
-        const float M = fabsf(c.ax) * fmaxf(fabsf(L[0]), fabsf(U[0])) + fabsf(c.ay) * fmaxf(fabsf(L[1]), fabsf(U[1])) +
-                        fabsf(c.az) * fmaxf(fabsf(L[2]), fabsf(U[2]));
+        const float S_ = fmaxf(c.ax * L_[0], c.ax * U_[0]) + fmaxf(c.ay * L_[1], c.ay * U_[1]) +
+                         fmaxf(c.az * L_[2], c.az * U_[2]);
+        const float M = fabsf(c.ax) * fmaxf(fabsf(L_[0]), fabsf(U_[0])) +
+                        fabsf(c.ay) * fmaxf(fabsf(L_[1]), fabsf(U_[1])) +
+                        fabsf(c.az) * fmaxf(fabsf(L_[2]), fabsf(U_[2]));
         return (c.H - S_) > kSlack * (c.pH + M) + kTiny32;
     }
 
-    // Same two node bounds as the tier-1 cell (octant corner radius and
-    // per-axis support); see Cell::node_prunable.
+    // Node prune.  Every point p of the box has a = p - s with |a| >= d and
+    // H = |a|^2 - (w_p - w_s) >= |a|^2 - dw (dw = w_max - w_s).  Two bounds
+    // on the cell side, either of which suffices:
+    //  * octant radius (voronoi_bvh.cuh:71-107, pwr_bvh.cuh:150-185): a . y
+    //    <= |a| rho with rho the largest AABB corner radius over the octants
+    //    the box overlaps, so no cut when |a|^2 - rho |a| - dw >= 0 for all
+    //    |a| >= d, i.e. 2d >= rho and (d^2 - dw)^2 >= rho^2 d^2, d^2 > dw;
+    //  * support: a . y <= sum_k max over the box's faces of a_k L_k, a_k U_k
+    //    (separable per axis), against the lower bound d^2 - dw of H.
     __device__ __forceinline__ bool node_prunable(float4 lo, float4 hi, float &key) const {
         const float alx = lo.x - sx, aly = lo.y - sy, alz = lo.z - sz;
         const float ahx = hi.x - sx, ahy = hi.y - sy, ahz = hi.z - sz;
@@ -334,36 +402,55 @@ struct Group {
         const float d2lo = d2 * (1.0f - kSlack);
         const float h = d2lo - dw - kSlack * (fabsf(lo.w) + fabsf(sw)) - kTiny32;
         if (4.0f * d2lo > rho2 && h > 0.0f && h * h > rho2 * d2 * (1.0f + kSlack)) return true;
-        const float S_ = fmaxf(fmaxf(alx * L[0], alx * U[0]), fmaxf(ahx * L[0], ahx * U[0])) +
-                         fmaxf(fmaxf(aly * L[1], aly * U[1]), fmaxf(ahy * L[1], ahy * U[1])) +
-                         fmaxf(fmaxf(alz * L[2], alz * U[2]), fmaxf(ahz * L[2], ahz * U[2]));
-        const float M = fmaxf(fabsf(alx), fabsf(ahx)) * fmaxf(fabsf(L[0]), fabsf(U[0])) +
-                        fmaxf(fabsf(aly), fabsf(ahy)) * fmaxf(fabsf(L[1]), fabsf(U[1])) +
-                        fmaxf(fabsf(alz), fabsf(ahz)) * fmaxf(fabsf(L[2]), fabsf(U[2]));
+        const float S_ = fmaxf(fmaxf(alx * L_[0], alx * U_[0]), fmaxf(ahx * L_[0], ahx * U_[0])) +
+                         fmaxf(fmaxf(aly * L_[1], aly * U_[1]), fmaxf(ahy * L_[1], ahy * U_[1])) +
+                         fmaxf(fmaxf(alz * L_[2], alz * U_[2]), fmaxf(ahz * L_[2], ahz * U_[2]));
+        const float M = fmaxf(fabsf(alx), fabsf(ahx)) * fmaxf(fabsf(L_[0]), fabsf(U_[0])) +
+                        fmaxf(fabsf(aly), fabsf(ahy)) * fmaxf(fabsf(L_[1]), fabsf(U_[1])) +
+                        fmaxf(fabsf(alz), fabsf(ahz)) * fmaxf(fabsf(L_[2]), fabsf(U_[2]));
         return (hlow - S_) > kSlack * (d2 + fabsf(lo.w) + fabsf(sw) + M) + kTiny32;
     }
 
     // ---------------------------------------------------------- planes
-    __device__ uint64_t used_planes() const {
-        uint64_t m = 0;
+    // Planes referenced by some vertex (group-uniform mask of P bits).
+    __device__ void used_planes(uint64_t (&u)[kPW]) const {
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            const int v = gl + kGL * k;
-            if ((valid >> v) & 1ull) {
-                const uint32_t t = S->vtri[v];
-                m |= (1ull << tri_get<8>(t, 0)) | (1ull << tri_get<8>(t, 1)) | (1ull << tri_get<8>(t, 2));
+        for (int w = 0; w < kPW; ++w) u[w] = 0;
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            if (own(valid, k)) {
+                const uint32_t t = S->vtri[gl + G * k];
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const uint32_t p = tri_get<kB>(t, e);
+#pragma unroll
+                    for (int w = 0; w < kPW; ++w) u[w] |= (int)(p >> 6) == w ? (1ull << (p & 63)) : 0ull;
+                }
             }
         }
-        return gor64(m);
+#pragma unroll
+        for (int w = 0; w < kPW; ++w) u[w] = gor64(u[w]);
     }
 
-    // Drops unreferenced planes; new index of plane p = popcount(used below p).
-    __device__ void garbage_collect() {
-        const uint64_t used = used_planes();
+    // new index of a kept plane: referenced planes below it
+    __device__ __forceinline__ uint32_t plane_rank(const uint64_t (&u)[kPW], uint32_t p) const {
+        int r = 0;
 #pragma unroll
-        for (int k = 0; k < (kGP + kGL - 1) / kGL; ++k) {
-            const int p = gl + kGL * k;
-            const bool mv = p < np && ((used >> p) & 1ull);
+        for (int w = 0; w < kPW; ++w) {
+            const uint64_t m = (int)(p >> 6) > w ? ~0ull : ((int)(p >> 6) == w ? ((1ull << (p & 63)) - 1ull) : 0ull);
+            r += __popcll(u[w] & m);
+        }
+        return (uint32_t)r;
+    }
+
+    // Drops unreferenced planes and renumbers the vertex triples.
+    __device__ __noinline__ void garbage_collect() {
+        uint64_t used[kPW];
+        used_planes(used);
+#pragma unroll
+        for (int k = 0; k < kPRows; ++k) {
+            const int p = gl + G * k;
+            const bool mv = p < np && mask_bit(used, p);
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
             float2 b = make_float2(0.f, 0.f);
             if (mv) {
@@ -371,35 +458,31 @@ struct Group {
                 b = S->plx[p];
             }
             // every lane has read slot p before any lane writes a slot below
-            // 16 (k + 1): all later reads are at or above that
+            // G (k + 1): all later reads are at or above that
             if (mv) {
-                const int dst = __popcll(used & ((1ull << p) - 1ull));
+                const uint32_t dst = plane_rank(used, (uint32_t)p);
                 S->pl[dst] = a;
                 S->plx[dst] = b;
             }
         }
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            const int v = gl + kGL * k;
-            if ((valid >> v) & 1ull) {
+        for (int k = 0; k < kRows; ++k) {
+            if (own(valid, k)) {
+                const int v = gl + G * k;
                 const uint32_t t = S->vtri[v];
-                uint32_t r[3];
-                for (int e = 0; e < 3; ++e) {
-                    const uint32_t p = tri_get<8>(t, e);
-                    r[e] = (uint32_t)__popcll(used & ((1ull << p) - 1ull));
-                }
-                S->vtri[v] = tri_pack<8>(r[0], r[1], r[2]);
+                S->vtri[v] = tri_pack<kB>(plane_rank(used, tri_get<kB>(t, 0)), plane_rank(used, tri_get<kB>(t, 1)),
+                                         plane_rank(used, tri_get<kB>(t, 2)));
             }
         }
-        np = __popcll(used);
+        np = mask_popc(used);
     }
 
     __device__ bool has_plane(int owner) const {
         if (!(bloom & bloom_bit((uint32_t)owner))) return false;
         bool hit = false;
 #pragma unroll
-        for (int k = 0; k < (kGP + kGL - 1) / kGL; ++k) {
-            const int p = gl + kGL * k;
+        for (int k = 0; k < kPRows; ++k) {
+            const int p = gl + G * k;
             hit |= p < np && __float_as_int(S->plx[p].y) == owner;
         }
         return gany(hit);
@@ -408,23 +491,25 @@ struct Group {
     // ---------------------------------------------------------- clipping
     // Offers plane q (group-uniform).  Returns true when the cell changed.
     __device__ bool clip(const Cand &q) {
-        uint64_t rem = 0;
+        uint64_t rem[kW];
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            if (((valid >> (kGL * k)) & (uint64_t)kGMask) == 0ull) continue;
-            const int v = gl + kGL * k;
-            const bool r = ((valid >> v) & 1ull) && beyond(v, q);
-            rem |= (uint64_t)gballot(r) << (kGL * k);
+        for (int w = 0; w < kW; ++w) rem[w] = 0;
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            if (row(valid, k) == 0ull) continue;
+            const bool r = own(valid, k) && beyond(gl + G * k, q);
+            rem[(G * k) >> 6] |= gballot(r) << ((G * k) & 63);
         }
-        if (rem == 0ull) return false;
-        const int R = __popcll(rem);
-        if (R == __popcll(valid)) {  // voronoi_convex_cell.cuh:430-435
+        const int R = mask_popc(rem);
+        if (R == 0) return false;
+        if (R == mask_popc(valid)) {  // voronoi_convex_cell.cuh:430-435
             status = kEmptyCell;
-            valid = 0ull;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) valid[w] = 0;
             return true;
         }
-        if (np >= kGP) garbage_collect();
-        if (np >= kGP) {
+        if (np >= P) garbage_collect();
+        if (np >= P) {
             overflow = true;
             return false;
         }
@@ -434,113 +519,104 @@ struct Group {
             S->plx[qs] = make_float2(q.pH, __int_as_float(q.owner));
         }
         if (in_guess) bloom |= bloom_bit((uint32_t)q.owner);
+        if (R > G) return clip_rows(rem, qs);
         // Boundary of the removed triangles: directed edges whose twin is not
         // removed.  The R removed triangles are spread one per lane (through
         // tmp[]), and each lane checks its three edges against all of them.
-        if (R > kGL) return clip_wide(q, rem, qs);
         int base = 0;
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            const uint32_t bits = (uint32_t)(rem >> (kGL * k)) & kGMask;
-            if ((bits >> gl) & 1u) S->tmp[base + __popc(bits & ((1u << gl) - 1u))] = S->vtri[gl + kGL * k];
-            base += __popc(bits);
+        for (int k = 0; k < kRows; ++k) {
+            const uint64_t bits = row(rem, k);
+            if ((bits >> gl) & 1ull) S->tmp[base + __popcll(bits & ((1ull << gl) - 1ull))] = S->vtri[gl + G * k];
+            base += __popcll(bits);
         }
-        const bool mine_t = gl < R;
-        const uint32_t tr = mine_t ? S->tmp[gl] : 0u;
+        const bool mine = gl < R;
+        const uint32_t tr = mine ? S->tmp[gl] : 0u;
         uint32_t edge[3];
         uint32_t emask = 0;
         int sx_ = 0, sy_ = 0;
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
-            const uint32_t x = tri_get<8>(tr, e), y = tri_get<8>(tr, (e + 1) % 3);
+            const uint32_t x = tri_get<kB>(tr, e), y = tri_get<kB>(tr, (e + 1) % 3);
             bool twin = false;
-            for (int u = 0; u < R; ++u) twin |= tri_has_edge<8>((uint32_t)__shfl((int)tr, u, kGL), y, x);
-            edge[e] = x | (y << 8);
-            if (mine_t && !twin) {
+            for (int u = 0; u < R; ++u) twin |= tri_has_edge<kB>((uint32_t)__shfl((int)tr, u, G), y, x);
+            edge[e] = x | (y << kB);
+            if (mine && !twin) {
                 emask |= 1u << e;
                 sx_ += (int)(x * x + 1u);
                 sy_ += (int)(y * y + 1u);
             }
         }
+        return finish_clip(rem, qs, edge, emask, sx_, sy_);
+    }
+
+    // Boundary pass for clips that remove more triangles than the group has
+    // lanes (early clips of the initial box): each lane walks its own rows.
+    __device__ __noinline__ bool clip_rows(const uint64_t (&rem)[kW], int qs) {
+        uint32_t edge[kRows * 3];
+        uint64_t emask = 0;
+        int sx_ = 0, sy_ = 0;
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            const bool r = own(rem, k);
+            const uint32_t t = r ? S->vtri[gl + G * k] : 0u;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                edge[3 * k + e] = 0;
+                if (!r) continue;
+                const uint32_t x = tri_get<kB>(t, e), y = tri_get<kB>(t, (e + 1) % 3);
+                bool twin = false;
+#pragma unroll
+                for (int w = 0; w < kW; ++w) {
+                    uint64_t m = twin ? 0ull : rem[w];
+                    while (m && !twin) {
+                        const int u = 64 * w + (int)__builtin_ctzll(m);
+                        m &= m - 1ull;
+                        twin = tri_has_edge<kB>(S->vtri[u], y, x);
+                    }
+                }
+                if (!twin) {
+                    emask |= 1ull << (3 * k + e);
+                    edge[3 * k + e] = x | (y << kB);
+                    sx_ += (int)(x * x + 1u);
+                    sy_ += (int)(y * y + 1u);
+                }
+            }
+        }
+        return finish_clip(rem, qs, edge, emask, sx_, sy_);
+    }
+
+    // Places one new vertex (q, x, next[x]) per boundary edge into the free
+    // slots (lowest first) and updates the valid mask.
+    template <int N>
+    __device__ bool finish_clip(const uint64_t (&rem)[kW], int qs, const uint32_t (&edge)[N], uint64_t emask, int sx_,
+                                int sy_) {
+        static_assert(N <= 64, "edge mask is 64 bits");
         int nb;
-        int j = gexcl(__popc(emask), nb);
+        int j = gexcl(__popcll(emask), nb);
         // every plane of the boundary cycle starts one edge and ends one
         if (nb < 3 || gsum(sx_) != gsum(sy_)) {
             status = kInconsistentBoundary;
             if (gl == 0) c_inconsistent++;
             return true;
         }
-        const uint64_t freem = ~(valid & ~rem);
-        if (nb > __popcll(freem)) {
+        uint64_t freem[kW];
+#pragma unroll
+        for (int w = 0; w < kW; ++w) freem[w] = ~(valid[w] & ~rem[w]);
+        if (nb > mask_popc(freem)) {
             overflow = true;
             return false;
         }
-        uint64_t newbits = 0;
 #pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            if (!((emask >> e) & 1u)) continue;
-            const int slot = select_bit(freem, j++);
-            make_vertex(slot, (uint32_t)qs, edge[e] & 255u, edge[e] >> 8);
-            newbits |= 1ull << slot;
+        for (int t = 0; t < N; ++t) {
+            if (!((emask >> t) & 1ull)) continue;
+            const int slot = mask_select(freem, j++);
+            make_vertex(slot, (uint32_t)qs, edge[t] & kBMask, edge[t] >> kB);
         }
-        valid = (valid & ~rem) | gor64(newbits);
-        dirty = true;
-        return true;
-    }
-
-    // Boundary pass for clips that remove more triangles than a group has
-    // lanes (early clips of the initial box): each lane walks its own rows.
-    __device__ bool clip_wide(const Cand &q, uint64_t rem, int qs) {
-        (void)q;
-        uint32_t edge[kGV / kGL * 3];
-        uint32_t emask = 0;
-        int sx_ = 0, sy_ = 0;
+        uint64_t added[kW];
+        mask_first(freem, nb, added);
 #pragma unroll
-        for (int k = 0; k < kGV / kGL; ++k) {
-            const int v = gl + kGL * k;
-            const bool r = (rem >> v) & 1ull;
-            const uint32_t t = r ? S->vtri[v] : 0u;
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                edge[3 * k + e] = 0;
-                if (!r) continue;
-                const uint32_t x = tri_get<8>(t, e), y = tri_get<8>(t, (e + 1) % 3);
-                bool twin = false;
-                uint64_t m = rem;
-                while (m && !twin) {
-                    const int u = (int)__builtin_ctzll(m);
-                    m &= m - 1ull;
-                    twin = tri_has_edge<8>(S->vtri[u], y, x);
-                }
-                if (!twin) {
-                    emask |= 1u << (3 * k + e);
-                    edge[3 * k + e] = x | (y << 8);
-                    sx_ += (int)(x * x + 1u);
-                    sy_ += (int)(y * y + 1u);
-                }
-            }
-        }
-        int nb;
-        int j = gexcl(__popc(emask), nb);
-        if (nb < 3 || gsum(sx_) != gsum(sy_)) {
-            status = kInconsistentBoundary;
-            if (gl == 0) c_inconsistent++;
-            return true;
-        }
-        const uint64_t freem = ~(valid & ~rem);
-        if (nb > __popcll(freem)) {
-            overflow = true;
-            return false;
-        }
-        uint64_t newbits = 0;
-#pragma unroll
-        for (int t = 0; t < kGV / kGL * 3; ++t) {
-            if (!((emask >> t) & 1u)) continue;
-            const int slot = select_bit(freem, j++);
-            make_vertex(slot, (uint32_t)qs, edge[t] & 255u, edge[t] >> 8);
-            newbits |= 1ull << slot;
-        }
-        valid = (valid & ~rem) | gor64(newbits);
+        for (int w = 0; w < kW; ++w) valid[w] = (valid[w] & ~rem[w]) | added[w];
         dirty = true;
         return true;
     }
@@ -564,21 +640,22 @@ struct Group {
         return !too_far(c);
     }
 
-    // Offers a batch of up to 16 candidates (lane-specific j when `has`).
+    // Offers a batch of up to G candidates (lane-specific j when `has`).
     __device__ void offer_batch(bool has, uint32_t j) {
         Cand c;
         bool emptied = false;
-        bool live = has && prepare(j, c, emptied);
+        const bool live_c = has && prepare(j, c, emptied);
         if (gany(emptied)) {
             status = kEmptyCell;
-            valid = 0ull;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) valid[w] = 0;
             return;
         }
-        if (gl == 0) c_skipped += __popc(gballot(has && !live));
-        uint32_t todo = gballot(live);
+        if (gl == 0) c_skipped += __popcll(gballot(has && !live_c));
+        uint64_t todo = gballot(live_c);
         while (todo) {
-            const int b = __ffs(todo) - 1;
-            todo &= todo - 1u;
+            const int b = (int)__builtin_ctzll(todo);
+            todo &= todo - 1ull;
             Cand q;
             q.ax = gbf(c.ax, b);
             q.ay = gbf(c.ay, b);
@@ -616,6 +693,7 @@ struct Group {
             S->plx[gl] = make_float2(fabsf(h), __int_as_float(-1 - gl));
         }
         np = 6;
+        // the 8 corners go to slots 0..7 (lanes 0..7 of row 0)
         if (gl < 8) {
             const int cx = gl >> 2 & 1, cy = gl >> 1 & 1, cz = gl & 1;
             const uint32_t X = cx, Y = 2 + cy, Z = 4 + cz;
@@ -623,7 +701,8 @@ struct Group {
             if (sgn > 0) make_vertex(gl, X, Y, Z);
             else make_vertex(gl, X, Z, Y);
         }
-        valid = 0xFFull;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) valid[w] = w == 0 ? 0xFFull : 0ull;
         bloom = 0;
         update_bounds();
         dirty = false;
@@ -633,7 +712,7 @@ struct Group {
     __device__ void offer_leaf(uint32_t ref) {
         if (gl == 0) c_leaves++;
         const uint32_t f = leaf_first(ref), c = leaf_count(ref);
-        for (uint32_t o = 0; o < c && status == kSuccess && !overflow; o += kGL) {
+        for (uint32_t o = 0; o < c && status == kSuccess && !overflow; o += G) {
             const uint32_t j = f + o + (uint32_t)gl;
             offer_batch(o + (uint32_t)gl < c && j != seed, j);
         }
@@ -644,7 +723,7 @@ struct Group {
         float k = FLT_MAX;
         int idx = -1;
 #pragma unroll
-        for (int j = 0; j < kWList / kWave; ++j) {
+        for (int j = 0; j < L / kWave; ++j) {
             const int t = lane64 + kWave * j;
             if (t < sz) {
                 const float kt = W->key[t];
@@ -666,27 +745,27 @@ struct Group {
         return idx;
     }
 
-    // Wave-wide: does any group still need this box?  key: smallest bound.
+    // Wave-wide: does any cell of the wave still need this box?  key: the
+    // smallest bound over the cells that need it.
     __device__ bool packet_needs(float4 lo, float4 hi, float &key) const {
         float k = FLT_MAX;
         const bool need = live() && !node_prunable(lo, hi, k);
         const bool any = __ballot(gl == 0 && need) != 0ull;
         float kk = need ? k : FLT_MAX;
-        for (int o = kGL; o < kWave; o <<= 1) kk = fminf(kk, __shfl_xor(kk, o));
+        for (int o = G; o < kWave; o <<= 1) kk = fminf(kk, __shfl_xor(kk, o));
         key = kk;
         return any;
     }
 
-    // Best-first packet traversal: the four cells of a wave (Morton
-    // neighbours) walk the hierarchy together.  A box is expanded while any
-    // of them may still be cut by a point in it; its key is the smallest
-    // lower bound of H over the cells that need it; leaves are offered to
-    // every cell, each filtering and clipping on its own lanes.  Unpruned
-    // children wait in one wave-wide keyed list and the wave always goes on
-    // with the smallest key (the order voronoi_bvh.cuh:246-283 keeps with its
-    // min-extraction pops), re-testing a waiting box when it comes out.  A
-    // full list sends the wave's cells to the large-capacity tier instead of
-    // dropping entries.
+    // Best-first packet traversal: the cells of a wave (Morton neighbours)
+    // walk the hierarchy together.  A box is expanded while any of them may
+    // still be cut by a point in it; its key is the smallest lower bound of
+    // H over the cells that need it; leaves are offered to every cell, each
+    // filtering and clipping on its own lanes.  Unpruned children wait in one
+    // wave-wide keyed list and the wave always goes on with the smallest key
+    // (the order voronoi_bvh.cuh:246-283 keeps with its min-extraction pops),
+    // re-testing a waiting box when it comes out.  A full list sends the
+    // wave's cells to the next tier instead of dropping entries.
     __device__ void traverse() {
         uint32_t ref = root_ref;
         int sz = 0;
@@ -705,7 +784,7 @@ struct Group {
                     const bool near0 = !n1 || (n0 && k0 <= k1);
                     const float knear = near0 ? k0 : k1;
                     if (n0 && n1) {  // the farther child waits
-                        if (sz >= kWList) {
+                        if (sz >= L) {
                             overflow = true;
                             return;
                         }
@@ -721,7 +800,7 @@ struct Group {
                         ref = near0 ? r0 : r1;
                         have = true;
                     } else {  // a waiting box is nearer: the near child waits too
-                        if (sz >= kWList) {
+                        if (sz >= L) {
                             overflow = true;
                             return;
                         }
@@ -759,27 +838,28 @@ struct Group {
     // ---------------------------------------------------------- output
     // Row = sorted original ids of the owners of referenced planes
     // (voronoi_ultra.cu:125-138).  Returns false when the row does not fit.
-    __device__ bool emit(const CellParams &prm, uint32_t orig, int32_t *row) {
+    __device__ bool emit(const CellParams &prm, uint32_t orig, int32_t *row_out, int row_cap) {
         int F = 0;
         if (status != kEmptyCell) {
-            const uint64_t used = used_planes();
+            uint64_t used[kPW];
+            used_planes(used);
             if (prm.hdr->has_duplicates != 0u) {
                 // every exact duplicate of an owner carries the same face
                 bool full = false;
                 if (gl == 0) {
                     for (int p = 0; p < np; ++p) {
-                        if (!((used >> p) & 1ull)) continue;
+                        if (!mask_bit(used, p)) continue;
                         const int owner = __float_as_int(S->plx[p].y);
                         if (owner < 0) continue;
                         uint32_t m = prm.dup_head[owner];
                         while (m != 0xFFFFFFFFu) {
                             const int32_t id = (int32_t)perm[m];
                             int k = F;
-                            while (k > 0 && row[k - 1] > id) k--;
-                            if (!(k > 0 && row[k - 1] == id)) {
-                                if (F < kGP) {
-                                    for (int t = F; t > k; --t) row[t] = row[t - 1];
-                                    row[k] = id;
+                            while (k > 0 && row_out[k - 1] > id) k--;
+                            if (!(k > 0 && row_out[k - 1] == id)) {
+                                if (F < row_cap) {
+                                    for (int t = F; t > k; --t) row_out[t] = row_out[t - 1];
+                                    row_out[k] = id;
                                     F++;
                                 } else {
                                     full = true;
@@ -792,13 +872,13 @@ struct Group {
                 if (gany(full)) return false;
                 F = gbi(F, 0);
             } else {
-                int32_t ids[(kGP + kGL - 1) / kGL];
+                int32_t ids[kPRows];
                 int mine = 0;
 #pragma unroll
-                for (int k = 0; k < (kGP + kGL - 1) / kGL; ++k) {
-                    const int p = gl + kGL * k;
+                for (int k = 0; k < kPRows; ++k) {
+                    const int p = gl + G * k;
                     ids[k] = -1;
-                    if (p < np && ((used >> p) & 1ull)) {
+                    if (p < np && mask_bit(used, p)) {
                         const int owner = __float_as_int(S->plx[p].y);
                         if (owner >= 0) {
                             ids[k] = (int32_t)perm[owner];
@@ -807,16 +887,17 @@ struct Group {
                     }
                 }
                 int at = gexcl(mine, F);
+                if (F > row_cap) return false;
 #pragma unroll
-                for (int k = 0; k < (kGP + kGL - 1) / kGL; ++k)
+                for (int k = 0; k < kPRows; ++k)
                     if (ids[k] >= 0) S->tmp[at++] = (uint32_t)ids[k];
                 // rank of each id among the row (ids are distinct)
 #pragma unroll
-                for (int k = 0; k < (kGP + kGL - 1) / kGL; ++k) {
+                for (int k = 0; k < kPRows; ++k) {
                     if (ids[k] < 0) continue;
                     int rank = 0;
                     for (int t = 0; t < F; ++t) rank += (int32_t)S->tmp[t] < ids[k];
-                    row[rank] = ids[k];
+                    row_out[rank] = ids[k];
                 }
             }
         }
@@ -840,12 +921,15 @@ struct Group {
             // Morton neighbours first: the cheap analogue of clipping by the
             // k nearest neighbours (voronoi_ultra.cu:70-85); order does not
             // change the result, only how fast the cell shrinks
-            const int64_t j = (int64_t)cell - kGL / 2 + gl + (gl >= kGL / 2 ? 1 : 0);
-            offer_batch(gl < prm.seed_window && j >= 0 && j < prm.n, (uint32_t)(j < 0 ? 0 : j));
+            for (int o = 0; o < prm.seed_window && status == kSuccess && !overflow; o += G) {
+                const int i = o + gl;
+                const int64_t j = (int64_t)cell - prm.seed_window / 2 + i + (i >= prm.seed_window / 2 ? 1 : 0);
+                offer_batch(i < prm.seed_window && j >= 0 && j < prm.n, (uint32_t)(j < 0 ? 0 : j));
+            }
         }
         if (prm.guesses) {
             const int32_t g0 = prm.guess_offsets[orig], g1 = prm.guess_offsets[orig + 1];
-            for (int32_t g = g0; g < g1 && status == kSuccess && !overflow; g += kGL) {
+            for (int32_t g = g0; g < g1 && status == kSuccess && !overflow; g += G) {
                 bool has = g + gl < g1;
                 uint32_t j = 0;
                 if (has) {
@@ -860,16 +944,77 @@ struct Group {
         in_guess = false;
     }
 
-    __device__ void finish(uint32_t cell, const CellParams &prm) {
-        const uint32_t orig = perm[cell];
-        const int64_t local = (int64_t)cell - prm.begin;
-        if (!overflow) {
-            if (gl == 0) prm.row_offsets[cell] = local * prm.row_stride;
-            if (emit(prm, orig, prm.rows + local * prm.row_stride)) return;
+    __device__ void setup(const CellParams &prm, GroupLDS<V, P> *s, WaveList<L> *w) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int grp = lane / G;
+        S = s;
+        W = w;
+        gl = lane & (G - 1);
+        lane64 = lane;
+        gshift = grp * G;
+        pts = prm.pts;
+        perm = prm.perm;
+        inv = prm.inv;
+        nodes = prm.nodes;
+        root_ref = prm.hdr->root_ref;
+        for (int k = 0; k < 3; ++k) box_lo[k] = prm.hdr->box_lo[k], box_hi[k] = prm.hdr->box_hi[k];
+        c_recheck = c_unresolved = c_inconsistent = c_overflow = c_clips = c_cands = 0;
+        c_nodes = c_leaves = c_skipped = 0;
+    }
+
+    __device__ void flush(const CellParams &prm, uint32_t cells) const {
+        const uint32_t vals[12] = {c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands,
+                                   c_nodes, cells, 0, 0, c_leaves, c_skipped};
+        for (int k = 0; k < 12; ++k) {
+            const uint32_t s = wave_sum(vals[k]);
+            if (lane64 == 0 && s) atomicAdd(&prm.stats[k], (unsigned long long)s);
         }
-        // queue for the large-capacity tier
-        if (gl == 0) {
-            c_overflow++;
+    }
+};
+
+// ------------------------------------------------------------------ tiers
+constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Threads = 256;
+constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 256, kT1Threads = 128;
+constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 1024, kT2Threads = 64;
+static_assert(kT0Planes <= kRowCap && kT1Planes <= kTier1Row && kT2Planes <= kTier2Row, "rows fit their strides");
+using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List>;
+using Tier1 = Group<kT1Lanes, kT1Verts, kT1Planes, kT1List>;
+using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List>;
+
+// Tier 0: a wave holds four Morton-consecutive cells per round (pulled from
+// a global counter) and traverses for them together.  A cell that outgrows
+// the tier is queued for tier 1.
+__global__ void __launch_bounds__(kT0Threads) cell_group_kernel(const CellParams prm) {
+    constexpr int kCellsPerWave = kWave / kT0Lanes;
+    __shared__ GroupLDS<kT0Verts, kT0Planes> lds[kT0Threads / kT0Lanes];
+    __shared__ WaveList<kT0List> wlist[kT0Threads / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int grp = lane / kT0Lanes;
+    Tier0 g;
+    g.setup(prm, &lds[(threadIdx.x / kWave) * kCellsPerWave + grp], &wlist[threadIdx.x / kWave]);
+    uint32_t cells = 0;
+    const int64_t count = prm.end - prm.begin;
+    while (true) {
+        uint32_t c0 = 0;
+        if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kCellsPerWave);
+        c0 = (uint32_t)__shfl((int)c0, 0);
+        if ((int64_t)c0 >= count) break;
+        const uint64_t t0 = clock64();
+        const int64_t c = (int64_t)c0 + grp;
+        g.active = c < count;
+        const uint32_t cell = (uint32_t)(prm.begin + (g.active ? c : 0));
+        if (g.active) g.start(cell, prm);
+        g.traverse();
+        if (!g.active) continue;
+        const uint32_t orig = prm.perm[cell];
+        const int64_t local = (int64_t)cell - prm.begin;
+        bool done = false;
+        if (!g.overflow) {
+            if (g.gl == 0) prm.row_offsets[cell] = local * prm.row_stride;
+            done = g.emit(prm, orig, prm.rows + local * prm.row_stride, prm.row_stride);
+        }
+        if (!done && g.gl == 0) {  // queue for the large-capacity tier
+            g.c_overflow++;
             const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
             if (slot < prm.big_cap) {
                 prm.big_list[slot] = cell;
@@ -879,58 +1024,87 @@ struct Group {
                 prm.status[orig] = kPlaneOverflow;
             }
         }
+        if (prm.cycles && g.gl == 0) prm.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (g.gl == 0) cells++;
     }
-};
+    g.flush(prm, cells);
+}
 
-// Capacity tier 0: 16 lanes per cell, a wave holds four Morton-consecutive
-// cells per round (pulled from a global counter) and traverses for them
-// together; cell state in LDS.
-__global__ void __launch_bounds__(kGThreads) cell_group_kernel(const CellParams prm) {
-    __shared__ GroupLDS lds[kGCellsPerBlock];
-    __shared__ WaveList wlist[kGThreads / kWave];
+// Tier 1: one cell per wave, the queued cells; rows after the tier-0 slab.
+__global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
+    __shared__ GroupLDS<kT1Verts, kT1Planes> lds[kT1Threads / kWave];
+    __shared__ WaveList<kT1List> wlist[kT1Threads / kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int grp = lane / kGL;
-    Group g;
-    g.S = &lds[(threadIdx.x / kWave) * (kWave / kGL) + grp];
-    g.W = &wlist[threadIdx.x / kWave];
-    g.gl = lane & (kGL - 1);
-    g.lane64 = lane;
-    g.gshift = grp * kGL;
-    g.pts = prm.pts;
-    g.perm = prm.perm;
-    g.inv = prm.inv;
-    g.nodes = prm.nodes;
-    g.root_ref = prm.hdr->root_ref;
-    for (int k = 0; k < 3; ++k) g.box_lo[k] = prm.hdr->box_lo[k], g.box_hi[k] = prm.hdr->box_hi[k];
-    g.c_recheck = g.c_unresolved = g.c_inconsistent = g.c_overflow = g.c_clips = g.c_cands = 0;
-    g.c_nodes = g.c_leaves = g.c_skipped = 0;
-    uint32_t cells = 0;
-    const int64_t count = prm.end - prm.begin;
-    constexpr int kPacket = kWave / kGL;
+    Tier1 g;
+    g.setup(prm, &lds[threadIdx.x / kWave], &wlist[threadIdx.x / kWave]);
+    const uint32_t queued = min(prm.counters[1], prm.big_cap);
     while (true) {
-        uint32_t c0 = 0;
-        if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kPacket);
-        c0 = (uint32_t)__shfl((int)c0, 0);
-        if ((int64_t)c0 >= count) break;
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&prm.counters[2], 1u);
+        k = (uint32_t)__shfl((int)k, 0);
+        if (k >= queued) break;
         const uint64_t t0 = clock64();
-        const int64_t c = (int64_t)c0 + grp;
-        g.active = c < count;
-        const uint32_t cell = (uint32_t)(prm.begin + (g.active ? c : 0));
-        if (g.active) g.start(cell, prm);
+        const uint32_t cell = prm.big_list[k];
+        g.active = true;
+        g.start(cell, prm);
         g.traverse();
-        if (g.active) {
-            g.finish(cell, prm);
-            if (prm.cycles && g.gl == 0)
-                prm.cycles[prm.perm[cell]] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
-            if (g.gl == 0) cells++;
+        const uint32_t orig = prm.perm[cell];
+        const int64_t off = (int64_t)(prm.end - prm.begin) * prm.row_stride + (int64_t)k * kTier1Row;
+        if (lane == 0) prm.row_offsets[cell] = off;
+        if (g.overflow || !g.emit(prm, orig, prm.rows + off, kTier1Row)) {
+            if (lane == 0) {  // queue for tier 2
+                g.c_overflow++;
+                const uint32_t slot = atomicAdd(&prm.counters[3], 1u);
+                if (slot < prm.huge_cap) {
+                    prm.huge_list[slot] = cell;
+                } else {
+                    prm.counts[orig] = 0;
+                    prm.status[orig] = kPlaneOverflow;
+                }
+            }
         }
+        if (prm.cycles && lane == 0)
+            prm.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
     }
-    const uint32_t vals[12] = {g.c_recheck, g.c_unresolved, g.c_inconsistent, g.c_overflow, g.c_clips, g.c_cands,
-                               g.c_nodes, cells, 0, 0, g.c_leaves, g.c_skipped};
-    for (int k = 0; k < 12; ++k) {
-        const uint32_t s = wave_sum(vals[k]);
-        if (lane == 0 && s) atomicAdd(&prm.stats[k], (unsigned long long)s);
+    g.flush(prm, 0);
+}
+
+// Tier 2: one cell per wave, one wave per block (57 KB of LDS), the cells
+// that outgrow tier 1; rows after the tier-1 rows.
+__global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellParams prm) {
+    __shared__ GroupLDS<kT2Verts, kT2Planes> lds;
+    __shared__ WaveList<kT2List> wlist;
+    const int lane = threadIdx.x & (kWave - 1);
+    Tier2 g;
+    g.setup(prm, &lds, &wlist);
+    const uint32_t queued = min(prm.counters[3], prm.huge_cap);
+    while (true) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
+        k = (uint32_t)__shfl((int)k, 0);
+        if (k >= queued) break;
+        const uint64_t t0 = clock64();
+        const uint32_t cell = prm.huge_list[k];
+        g.active = true;
+        g.start(cell, prm);
+        g.traverse();
+        const uint32_t orig = prm.perm[cell];
+        const int64_t off = (int64_t)(prm.end - prm.begin) * prm.row_stride + (int64_t)prm.big_cap * kTier1Row +
+                            (int64_t)k * kTier2Row;
+        if (lane == 0) prm.row_offsets[cell] = off;
+        if (g.overflow || !g.emit(prm, orig, prm.rows + off, kTier2Row)) {
+            // beyond every tier: report the reference's capacity status with
+            // an empty row (counted, never silent)
+            if (lane == 0) {
+                g.c_overflow++;
+                prm.counts[orig] = 0;
+                prm.status[orig] = kPlaneOverflow;
+            }
+        }
+        if (prm.cycles && lane == 0)
+            prm.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
     }
+    g.flush(prm, 0);
 }
 
 }  // namespace pg

@@ -22,7 +22,7 @@ struct CellArgs {
     const int32_t *guess_offsets; // indexed by original id (null with guesses == null)
     int guesses_sorted;           // guesses already hold sorted indices (kNN output)
     int seed_window;              // Morton neighbours offered before the traversal
-    int32_t *rows;                // (end - begin) x row_stride, then big_cap x kTier1Row
+    int32_t *rows;                // (end - begin) x row_stride, big_cap x kTier1Row, huge_cap x kTier2Row
     int row_stride;
     int64_t *row_offsets;         // [n] sorted cell -> row start in rows
     int32_t *counts;              // [n] original order
@@ -33,10 +33,11 @@ struct CellArgs {
     int waves;                    // persistent tier-0 waves (scratch holds one slot per wave)
     int waves1;                   // persistent tier-1 waves
     int64_t big_cap;              // tier-1 queue capacity
+    int64_t huge_cap;             // tier-2 queue capacity
     uint32_t *cycles;             // optional per-cell cost counters
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
-uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap);
+uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap);
 
 // CSR helpers.
 hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int64_t begin, int64_t end,
