@@ -70,22 +70,31 @@ __device__ __forceinline__ bool mask_bit(const uint64_t (&m)[W], int v) {
     for (int w = 0; w < W; ++w) word = (v >> 6) == w ? m[w] : word;
     return (word >> (v & 63)) & 1ull;
 }
-// index of the j-th set bit (j < popcount)
+// index of the j-th set bit (j < popcount): the word by prefix counts, then
+// a binary search on popcounts within the word
 template <int W>
 __device__ __forceinline__ int mask_select(const uint64_t (&m)[W], int j) {
+    uint64_t x = m[0];
     int base = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        const int c = __popcll(m[w]);
-        if (j < c) {
-            uint64_t x = m[w];
-            for (int t = 0; t < j; ++t) x &= x - 1ull;
-            return base + (int)__builtin_ctzll(x);
+    for (int w = 1; w < W; ++w) {
+        const int c = __popcll(x);
+        if (j >= c) {
+            j -= c;
+            x = m[w];
+            base = 64 * w;
         }
-        j -= c;
-        base += 64;
     }
-    return -1;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const int c = __popcll(x & ((1ull << s) - 1ull));
+        if (j >= c) {
+            j -= c;
+            x >>= s;
+            base += s;
+        }
+    }
+    return base;
 }
 // the lowest n set bits of m
 template <int W>
@@ -101,6 +110,118 @@ __device__ __forceinline__ void mask_first(const uint64_t (&m)[W], int n, uint64
         }
         out[w] = keep;
     }
+}
+
+// ------------------------------------------------------------------ FP64
+// The FP64 recheck path, shared by all tiers.  Plane of owner o relative to
+// the seed s (doubled coordinates): a = p - s, H = |a|^2 - (w_p - w_s); box
+// planes a = -e_k, H = 2 (s_k - lo_k) and a = e_k, H = 2 (hi_k - s_k).  The
+// p-terms are the matching permanents (the same expressions in absolute
+// values), which bound the rounding error: |x~ - x| <= 1e-13 p for every
+// quantity below (a few dozen roundings at 2^-53 each).
+__device__ __forceinline__ void fp64_plane(const float4 *pts, const IndexHeader *hdr, int owner, double sx,
+                                           double sy, double sz, double sw, double a[3], double &H, double pa[3],
+                                           double &pH) {
+    if (owner >= 0) {
+        const float4 p = pts[owner];
+        a[0] = (double)p.x - sx;
+        a[1] = (double)p.y - sy;
+        a[2] = (double)p.z - sz;
+        const double dw = (double)p.w - sw;
+        const double a2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        H = a2 - dw;
+        pH = a2 + fabs(dw);
+        for (int k = 0; k < 3; ++k) pa[k] = fabs(a[k]);
+    } else {
+        const int code = -1 - owner, axis = code >> 1, side = code & 1;
+        const double s = axis == 0 ? sx : (axis == 1 ? sy : sz);
+        for (int k = 0; k < 3; ++k) {
+            a[k] = k == axis ? (side ? 1.0 : -1.0) : 0.0;
+            pa[k] = k == axis ? 1.0 : 0.0;
+        }
+        H = side ? 2.0 * ((double)hdr->box_hi[axis] - s) : 2.0 * (s - (double)hdr->box_lo[axis]);
+        pH = fabs(H);
+    }
+}
+
+// Vertex of planes (i, j, k): N = H_i (a_j x a_k) + H_j (a_k x a_i) + H_k (a_i x a_j),
+// D = a_i . (a_j x a_k), y = N / D, with permanents pN, pD.
+template <int kB>
+__device__ __forceinline__ void fp64_vertex(const float2 *plx, const float4 *pts, const IndexHeader *hdr,
+                                            uint32_t t, double sx, double sy, double sz, double sw, double N[3],
+                                            double &D, double pN[3], double &pD) {
+    double a[3][3], H[3], pa[3][3], pH[3];
+    for (int e = 0; e < 3; ++e) {
+        const int owner = __float_as_int(plx[tri_get<kB>(t, e)].y);
+        fp64_plane(pts, hdr, owner, sx, sy, sz, sw, a[e], H[e], pa[e], pH[e]);
+    }
+    double c[3][3], pc[3][3];
+    for (int e = 0; e < 3; ++e) {
+        const double *u = a[(e + 1) % 3], *v = a[(e + 2) % 3];
+        const double *pu = pa[(e + 1) % 3], *pv = pa[(e + 2) % 3];
+        c[e][0] = u[1] * v[2] - u[2] * v[1];
+        c[e][1] = u[2] * v[0] - u[0] * v[2];
+        c[e][2] = u[0] * v[1] - u[1] * v[0];
+        pc[e][0] = pu[1] * pv[2] + pu[2] * pv[1];
+        pc[e][1] = pu[2] * pv[0] + pu[0] * pv[2];
+        pc[e][2] = pu[0] * pv[1] + pu[1] * pv[0];
+    }
+    D = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
+    pD = pa[0][0] * pc[0][0] + pa[0][1] * pc[0][1] + pa[0][2] * pc[0][2];
+    for (int k = 0; k < 3; ++k) {
+        N[k] = H[0] * c[0][k] + H[1] * c[1][k] + H[2] * c[2][k];
+        pN[k] = pH[0] * pc[0][k] + pH[1] * pc[1][k] + pH[2] * pc[2][k];
+    }
+}
+
+// FP64 removal decision for the vertex with plane triple t against the
+// plane of point q_owner: P = a_q . N - H_q D > 0.  Returns bit 0: beyond,
+// bit 1: undecided even in FP64 (sign of the FP64 value taken), bit 2: the
+// vertex orientation D is certainly negative.
+template <int kB>
+__device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+                                         float sx_, float sy_, float sz_, float sw_, int q_owner) {
+    const double sx = sx_, sy = sy_, sz = sz_, sw = sw_;
+    double N[3], D, pN[3], pD;
+    fp64_vertex<kB>(plx, pts, hdr, t, sx, sy, sz, sw, N, D, pN, pD);
+    double a[3], H, pa[3], pH;
+    fp64_plane(pts, hdr, q_owner, sx, sy, sz, sw, a, H, pa, pH);
+    const double Pv = a[0] * N[0] + a[1] * N[1] + a[2] * N[2] - H * D;
+    const double pP = pa[0] * pN[0] + pa[1] * pN[1] + pa[2] * pN[2] + pH * pD;
+    int flags = 0;
+    if (fabs(D) > kG64 * pD && pD > kTiny64 && D < 0.0) flags |= 4;
+    if (fabs(Pv) > kG64 * pP && pP > kTiny64) return flags | (Pv > 0.0 ? 1 : 0);
+    // A point identical (position and weight) to the owner of one of the
+    // vertex's planes gives the same plane: P = 0 exactly, not beyond.
+    const float4 q = pts[q_owner];
+    for (int e = 0; e < 3; ++e) {
+        const int o = __float_as_int(plx[tri_get<kB>(t, e)].y);
+        if (o < 0) continue;
+        const float4 r = pts[o];
+        if (r.x == q.x && r.y == q.y && r.z == q.z && r.w == q.w) return flags;
+    }
+    return flags | 2 | (Pv > 0.0 ? 1 : 0);
+}
+
+// Position and error radius of a vertex whose FP32 orientation is not
+// safely positive; the radius is +inf when FP64 cannot bound it either
+// (every test on that vertex then goes to FP64).
+template <int kB>
+__device__ __noinline__ float4 fp64_position(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+                                             float sx, float sy, float sz, float sw) {
+    double N[3], D, pN[3], pD;
+    fp64_vertex<kB>(plx, pts, hdr, t, sx, sy, sz, sw, N, D, pN, pD);
+    const double e64 = kG64 * pD;
+    if (!(D > 2.0 * e64 && pD > kTiny64)) return make_float4(0.f, 0.f, 0.f, __int_as_float(0x7f800000));
+    float x[3];
+    float eps = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const double X = N[k] / D;
+        const double E = (kG64 * pN[k] + fabs(X) * e64) / (D - e64) * (1.0 + 1e-12) + fabs(X) * 1e-12;
+        x[k] = (float)X;
+        eps = fmaxf(eps, (float)E + fabsf(x[k]) * kSlack);
+    }
+    return make_float4(x[0], x[1], x[2], eps + kTiny32);
 }
 
 // ------------------------------------------------------------------ group
@@ -125,8 +246,8 @@ struct Group {
     const uint32_t *perm;
     const uint32_t *inv;
     const Node *nodes;
+    const IndexHeader *hdr;
     uint32_t root_ref;
-    float box_lo[3], box_hi[3];
 
     uint32_t seed;
     float sx, sy, sz, sw;
@@ -188,107 +309,23 @@ struct Group {
     }
 
     // ---------------------------------------------------------- FP64 path
-    __device__ void plane64(int owner, double a[3], double &H, double pa[3], double &pH) const {
-        const double s[3] = {(double)sx, (double)sy, (double)sz};
-        if (owner >= 0) {
-            const float4 p = pts[owner];
-            a[0] = (double)p.x - s[0];
-            a[1] = (double)p.y - s[1];
-            a[2] = (double)p.z - s[2];
-            const double dw = (double)p.w - (double)sw;
-            const double a2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-            H = a2 - dw;
-            pH = a2 + fabs(dw);
-            for (int k = 0; k < 3; ++k) pa[k] = fabs(a[k]);
-        } else {
-            const int code = -1 - owner, axis = code >> 1, side = code & 1;
-            for (int k = 0; k < 3; ++k) a[k] = 0.0, pa[k] = 0.0;
-            pa[axis] = 1.0;
-            if (side == 0) {
-                a[axis] = -1.0;
-                H = 2.0 * (s[axis] - (double)box_lo[axis]);
-            } else {
-                a[axis] = 1.0;
-                H = 2.0 * ((double)box_hi[axis] - s[axis]);
-            }
-            pH = fabs(H);
-        }
-    }
-
-    __device__ void vertex64(uint32_t t, double N[3], double &D, double pN[3], double &pD) const {
-        double a[3][3], H[3], pa[3][3], pH[3];
-        for (int e = 0; e < 3; ++e) {
-            const int owner = __float_as_int(S->plx[tri_get<kB>(t, e)].y);
-            plane64(owner, a[e], H[e], pa[e], pH[e]);
-        }
-        double c[3][3], pc[3][3];
-        for (int e = 0; e < 3; ++e) {
-            const double *u = a[(e + 1) % 3], *v = a[(e + 2) % 3];
-            const double *pu = pa[(e + 1) % 3], *pv = pa[(e + 2) % 3];
-            c[e][0] = u[1] * v[2] - u[2] * v[1];
-            c[e][1] = u[2] * v[0] - u[0] * v[2];
-            c[e][2] = u[0] * v[1] - u[1] * v[0];
-            pc[e][0] = pu[1] * pv[2] + pu[2] * pv[1];
-            pc[e][1] = pu[2] * pv[0] + pu[0] * pv[2];
-            pc[e][2] = pu[0] * pv[1] + pu[1] * pv[0];
-        }
-        D = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
-        pD = pa[0][0] * pc[0][0] + pa[0][1] * pc[0][1] + pa[0][2] * pc[0][2];
-        for (int k = 0; k < 3; ++k) {
-            N[k] = H[0] * c[0][k] + H[1] * c[1][k] + H[2] * c[2][k];
-            pN[k] = pH[0] * pc[0][k] + pH[1] * pc[1][k] + pH[2] * pc[2][k];
-        }
-    }
-
-    // FP64 removal decision for vertex v against the plane of point q_owner.
-    __device__ __noinline__ bool recheck64(int v, int q_owner) {
+    // (free functions below: a call that passed `this` would move the whole
+    // group state from registers to scratch memory)
+    __device__ __forceinline__ bool recheck64(int v, int q_owner) {
         c_recheck++;
-        const uint32_t t = S->vtri[v];
-        double N[3], D, pN[3], pD;
-        vertex64(t, N, D, pN, pD);
-        double a[3], H, pa[3], pH;
-        plane64(q_owner, a, H, pa, pH);
-        const double Pv = a[0] * N[0] + a[1] * N[1] + a[2] * N[2] - H * D;
-        const double pP = pa[0] * pN[0] + pa[1] * pN[1] + pa[2] * pN[2] + pH * pD;
-        if (fabs(D) > kG64 * pD && pD > kTiny64 && D < 0.0) c_inconsistent++;
-        if (fabs(Pv) > kG64 * pP && pP > kTiny64) return Pv > 0.0;
-        // A point identical (position and weight) to the owner of one of the
-        // vertex's planes gives the same plane: P = 0 exactly, not beyond.
-        const float4 q = pts[q_owner];
-        for (int e = 0; e < 3; ++e) {
-            const int o = __float_as_int(S->plx[tri_get<kB>(t, e)].y);
-            if (o < 0) continue;
-            const float4 r = pts[o];
-            if (r.x == q.x && r.y == q.y && r.z == q.z && r.w == q.w) return false;
-        }
-        c_unresolved++;
-        return Pv > 0.0;
-    }
-
-    // Position and error radius of a vertex whose FP32 orientation is not
-    // safely positive; the radius is +inf when FP64 cannot bound it either
-    // (every test on that vertex then goes to FP64).
-    __device__ __noinline__ float4 position64(uint32_t t) {
-        double N[3], D, pN[3], pD;
-        vertex64(t, N, D, pN, pD);
-        const double e64 = kG64 * pD;
-        if (!(D > 2.0 * e64 && pD > kTiny64)) return make_float4(0.f, 0.f, 0.f, __int_as_float(0x7f800000));
-        float x[3];
-        float eps = 0.0f;
-        for (int k = 0; k < 3; ++k) {
-            const double X = N[k] / D;
-            const double E = (kG64 * pN[k] + fabs(X) * e64) / (D - e64) * (1.0 + 1e-12) + fabs(X) * 1e-12;
-            x[k] = (float)X;
-            eps = fmaxf(eps, (float)E + fabsf(x[k]) * kSlack);
-        }
-        return make_float4(x[0], x[1], x[2], eps + kTiny32);
+        const int r = fp64_recheck<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw, q_owner);
+        if (r & 2) c_unresolved++;
+        if (r & 4) c_inconsistent++;
+        return (r & 1) != 0;
     }
 
     // ---------------------------------------------------------- vertices
     // Builds vertex (i, j, k) into `slot` (one lane).  With
     // |N - N~| <= g pN and |D - D~| <= g pD (g = 2e-6), the position error is
     // |y - N~/D~| <= (g pN + |y~| g pD) / (D~ - g pD), plus division rounding.
-    __device__ void make_vertex(int slot, uint32_t i, uint32_t j, uint32_t k) {
+    // Returns false when the FP32 orientation is not safely positive: the
+    // slot then holds radius -1 and fix_vertices() replaces it.
+    __device__ __forceinline__ bool make_vertex(int slot, uint32_t i, uint32_t j, uint32_t k) {
         const float4 A = S->pl[i], B = S->pl[j], C = S->pl[k];
         const float pHA = S->plx[i].x, pHB = S->plx[j].x, pHC = S->plx[k].x;
         const float c1x = B.y * C.z - B.z * C.y, c1y = B.z * C.x - B.x * C.z, c1z = B.x * C.y - B.y * C.x;
@@ -310,8 +347,9 @@ struct Group {
         const float pNz = pHA * q1z + pHB * q2z + pHC * q3z;
         const uint32_t tri = tri_pack<kB>(i, j, k);
         const float eD = kG32 * pD;
-        float4 pos;
-        if (D > 2.0f * eD && pD > kTiny32) {
+        float4 pos = make_float4(0.f, 0.f, 0.f, -1.0f);
+        const bool ok = D > 2.0f * eD && pD > kTiny32;
+        if (ok) {
             const float inv = 1.0f / D;
             const float den = (1.0f + kSlack) / (D - eD);
             const float x = Nx * inv, y = Ny * inv, z = Nz * inv;
@@ -319,27 +357,42 @@ struct Group {
             const float ey = (kG32 * pNy + fabsf(y) * eD) * den + fabsf(y) * kSlack;
             const float ez = (kG32 * pNz + fabsf(z) * eD) * den + fabsf(z) * kSlack;
             pos = make_float4(x, y, z, fmaxf(ex, fmaxf(ey, ez)) + kTiny32);
-        } else {
-            if (D < -eD && pD > kTiny32) c_inconsistent++;
-            pos = position64(tri);
+        } else if (D < -eD && pD > kTiny32) {
+            c_inconsistent++;
         }
         S->vpos[slot] = pos;
         S->vtri[slot] = tri;
+        return ok;
+    }
+
+    // FP64 positions for the slots of `m` that make_vertex left unresolved
+    // (one call site: the rare path keeps its registers to itself).
+    __device__ __forceinline__ void fix_vertices(const uint64_t (&m)[kW]) {
+        uint32_t pend = 0;
+#pragma unroll
+        for (int k = 0; k < kRows; ++k)
+            if (own(m, k) && S->vpos[gl + G * k].w < 0.0f) pend |= 1u << k;
+        while (pend) {
+            const int v = gl + G * (int)__builtin_ctz(pend);
+            pend &= pend - 1u;
+            S->vpos[v] = fp64_position<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw);
+        }
     }
 
     // Is vertex v strictly beyond the plane of q?  Filter: the error of
     // a~ . y~ - H~ against a . y - H is at most |a|_1 eps + 2e-6 (sum |a_k y_k| + pH).
-    __device__ __forceinline__ bool beyond(int v, const Cand &q) {
+    // Returns 1 (beyond), 0 (not beyond) or -1 (undecided: FP64 recheck).
+    __device__ __forceinline__ int beyond32(int v, const Cand &q) const {
         const float4 p = S->vpos[v];
         const float tx = q.ax * p.x, ty = q.ay * p.y, tz = q.az * p.z;
         const float Pv = tx + ty + tz - q.H;
         const float err = q.pa1 * p.w * (1.0f + kSlack) + kG32 * (fabsf(tx) + fabsf(ty) + fabsf(tz) + q.pH) + kTiny32;
-        if (fabsf(Pv) > err) return Pv > 0.0f;
-        return recheck64(v, q.owner);
+        if (fabsf(Pv) > err) return Pv > 0.0f ? 1 : 0;
+        return -1;
     }
 
     // ---------------------------------------------------------- bounds
-    __device__ void update_bounds() {
+    __device__ __forceinline__ void update_bounds() {
         float lo0 = FLT_MAX, lo1 = FLT_MAX, lo2 = FLT_MAX, hi0 = -FLT_MAX, hi1 = -FLT_MAX, hi2 = -FLT_MAX;
         bool ok = true;
 #pragma unroll
@@ -413,7 +466,7 @@ struct Group {
 
     // ---------------------------------------------------------- planes
     // Planes referenced by some vertex (group-uniform mask of P bits).
-    __device__ void used_planes(uint64_t (&u)[kPW]) const {
+    __device__ __forceinline__ void used_planes(uint64_t (&u)[kPW]) const {
 #pragma unroll
         for (int w = 0; w < kPW; ++w) u[w] = 0;
 #pragma unroll
@@ -444,7 +497,7 @@ struct Group {
     }
 
     // Drops unreferenced planes and renumbers the vertex triples.
-    __device__ __noinline__ void garbage_collect() {
+    __device__ __forceinline__ void garbage_collect() {
         uint64_t used[kPW];
         used_planes(used);
 #pragma unroll
@@ -477,7 +530,7 @@ struct Group {
         np = mask_popc(used);
     }
 
-    __device__ bool has_plane(int owner) const {
+    __device__ __forceinline__ bool has_plane(int owner) const {
         if (!(bloom & bloom_bit((uint32_t)owner))) return false;
         bool hit = false;
 #pragma unroll
@@ -490,16 +543,25 @@ struct Group {
 
     // ---------------------------------------------------------- clipping
     // Offers plane q (group-uniform).  Returns true when the cell changed.
-    __device__ bool clip(const Cand &q) {
+    __device__ __forceinline__ bool clip(const Cand &q) {
         uint64_t rem[kW];
 #pragma unroll
         for (int w = 0; w < kW; ++w) rem[w] = 0;
+        uint32_t mine = 0, unsure = 0;
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
-            if (row(valid, k) == 0ull) continue;
-            const bool r = own(valid, k) && beyond(gl + G * k, q);
-            rem[(G * k) >> 6] |= gballot(r) << ((G * k) & 63);
+            if (!own(valid, k)) continue;
+            const int b = beyond32(gl + G * k, q);
+            if (b > 0) mine |= 1u << k;
+            if (b < 0) unsure |= 1u << k;
         }
+        while (unsure) {  // one FP64 call site
+            const int k = (int)__builtin_ctz(unsure);
+            unsure &= unsure - 1u;
+            if (recheck64(gl + G * k, q.owner)) mine |= 1u << k;
+        }
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) rem[(G * k) >> 6] |= gballot((mine >> k) & 1u) << ((G * k) & 63);
         const int R = mask_popc(rem);
         if (R == 0) return false;
         if (R == mask_popc(valid)) {  // voronoi_convex_cell.cuh:430-435
@@ -519,79 +581,39 @@ struct Group {
             S->plx[qs] = make_float2(q.pH, __int_as_float(q.owner));
         }
         if (in_guess) bloom |= bloom_bit((uint32_t)q.owner);
-        if (R > G) return clip_rows(rem, qs);
         // Boundary of the removed triangles: directed edges whose twin is not
-        // removed.  The R removed triangles are spread one per lane (through
-        // tmp[]), and each lane checks its three edges against all of them.
-        int base = 0;
-#pragma unroll
-        for (int k = 0; k < kRows; ++k) {
-            const uint64_t bits = row(rem, k);
-            if ((bits >> gl) & 1ull) S->tmp[base + __popcll(bits & ((1ull << gl) - 1ull))] = S->vtri[gl + G * k];
-            base += __popcll(bits);
-        }
-        const bool mine = gl < R;
-        const uint32_t tr = mine ? S->tmp[gl] : 0u;
-        uint32_t edge[3];
-        uint32_t emask = 0;
-        int sx_ = 0, sy_ = 0;
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            const uint32_t x = tri_get<kB>(tr, e), y = tri_get<kB>(tr, (e + 1) % 3);
-            bool twin = false;
-            for (int u = 0; u < R; ++u) twin |= tri_has_edge<kB>((uint32_t)__shfl((int)tr, u, G), y, x);
-            edge[e] = x | (y << kB);
-            if (mine && !twin) {
-                emask |= 1u << e;
-                sx_ += (int)(x * x + 1u);
-                sy_ += (int)(y * y + 1u);
-            }
-        }
-        return finish_clip(rem, qs, edge, emask, sx_, sy_);
-    }
-
-    // Boundary pass for clips that remove more triangles than the group has
-    // lanes (early clips of the initial box): each lane walks its own rows.
-    __device__ __noinline__ bool clip_rows(const uint64_t (&rem)[kW], int qs) {
-        uint32_t edge[kRows * 3];
+        // removed; bit 3r + e of emask marks edge e of the lane's r-th
+        // removed triangle.
         uint64_t emask = 0;
         int sx_ = 0, sy_ = 0;
+        uint32_t tr = 0u;
+        const bool spread = R <= G;
+        if (spread) {
+            // one removed triangle per lane (through tmp[]), each lane checks
+            // its three edges against all of them
+            int base = 0;
 #pragma unroll
-        for (int k = 0; k < kRows; ++k) {
-            const bool r = own(rem, k);
-            const uint32_t t = r ? S->vtri[gl + G * k] : 0u;
+            for (int k = 0; k < kRows; ++k) {
+                const uint64_t bits = row(rem, k);
+                if ((bits >> gl) & 1ull) S->tmp[base + __popcll(bits & ((1ull << gl) - 1ull))] = S->vtri[gl + G * k];
+                base += __popcll(bits);
+            }
+            const bool has_tri = gl < R;
+            tr = has_tri ? S->tmp[gl] : 0u;
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-                edge[3 * k + e] = 0;
-                if (!r) continue;
-                const uint32_t x = tri_get<kB>(t, e), y = tri_get<kB>(t, (e + 1) % 3);
+                const uint32_t x = tri_get<kB>(tr, e), y = tri_get<kB>(tr, e == 2 ? 0 : e + 1);
                 bool twin = false;
-#pragma unroll
-                for (int w = 0; w < kW; ++w) {
-                    uint64_t m = twin ? 0ull : rem[w];
-                    while (m && !twin) {
-                        const int u = 64 * w + (int)__builtin_ctzll(m);
-                        m &= m - 1ull;
-                        twin = tri_has_edge<kB>(S->vtri[u], y, x);
-                    }
-                }
-                if (!twin) {
-                    emask |= 1ull << (3 * k + e);
-                    edge[3 * k + e] = x | (y << kB);
+                for (int u = 0; u < R; ++u) twin |= tri_has_edge<kB>((uint32_t)__shfl((int)tr, u, G), y, x);
+                if (has_tri && !twin) {
+                    emask |= 1ull << e;
                     sx_ += (int)(x * x + 1u);
                     sy_ += (int)(y * y + 1u);
                 }
             }
+        } else {
+            boundary_rows(rem, emask, sx_, sy_);  // early clips of the initial box
         }
-        return finish_clip(rem, qs, edge, emask, sx_, sy_);
-    }
-
-    // Places one new vertex (q, x, next[x]) per boundary edge into the free
-    // slots (lowest first) and updates the valid mask.
-    template <int N>
-    __device__ bool finish_clip(const uint64_t (&rem)[kW], int qs, const uint32_t (&edge)[N], uint64_t emask, int sx_,
-                                int sy_) {
-        static_assert(N <= 64, "edge mask is 64 bits");
         int nb;
         int j = gexcl(__popcll(emask), nb);
         // every plane of the boundary cycle starts one edge and ends one
@@ -607,22 +629,62 @@ struct Group {
             overflow = true;
             return false;
         }
-#pragma unroll
-        for (int t = 0; t < N; ++t) {
-            if (!((emask >> t) & 1ull)) continue;
-            const int slot = mask_select(freem, j++);
-            make_vertex(slot, (uint32_t)qs, edge[t] & kBMask, edge[t] >> kB);
+        // compact the boundary edges into tmp[0, nb) (after every lane has
+        // read its triangle above)
+        while (emask) {
+            const int bit = (int)__builtin_ctzll(emask);
+            emask &= emask - 1ull;
+            const int k = bit / 3, e = bit - 3 * k;
+            const uint32_t t = spread ? tr : S->vtri[gl + G * k];
+            S->tmp[j++] = tri_get<kB>(t, e) | (tri_get<kB>(t, e == 2 ? 0 : e + 1) << kB);
+        }
+        // one new vertex (q, x, y) per boundary edge, in the free slots
+        // (lowest first), lane-strided
+        bool bad = false;
+        for (int e = gl; e < nb; e += G) {
+            const uint32_t ed = S->tmp[e];
+            bad |= !make_vertex(mask_select(freem, e), (uint32_t)qs, ed & kBMask, ed >> kB);
         }
         uint64_t added[kW];
         mask_first(freem, nb, added);
+        if (gany(bad)) fix_vertices(added);
 #pragma unroll
         for (int w = 0; w < kW; ++w) valid[w] = (valid[w] & ~rem[w]) | added[w];
         dirty = true;
         return true;
     }
 
+    // Boundary edges of clips that remove more triangles than the group has
+    // lanes: each lane walks its own removed rows.
+    __device__ __forceinline__ void boundary_rows(const uint64_t (&rem)[kW], uint64_t &emask, int &sx_, int &sy_) const {
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            if (!own(rem, k)) continue;
+            const uint32_t t = S->vtri[gl + G * k];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const uint32_t x = tri_get<kB>(t, e), y = tri_get<kB>(t, e == 2 ? 0 : e + 1);
+                bool twin = false;
+#pragma unroll
+                for (int w = 0; w < kW; ++w) {
+                    uint64_t m = twin ? 0ull : rem[w];
+                    while (m && !twin) {
+                        const int u = 64 * w + (int)__builtin_ctzll(m);
+                        m &= m - 1ull;
+                        twin = tri_has_edge<kB>(S->vtri[u], y, x);
+                    }
+                }
+                if (!twin) {
+                    emask |= 1ull << (3 * k + e);
+                    sx_ += (int)(x * x + 1u);
+                    sy_ += (int)(y * y + 1u);
+                }
+            }
+        }
+    }
+
     // Prepares lane's candidate j; false when it cannot cut the cell.
-    __device__ bool prepare(uint32_t j, Cand &c, bool &emptied) const {
+    __device__ __forceinline__ bool prepare(uint32_t j, Cand &c, bool &emptied) const {
         const float4 p = pts[j];
         c.ax = p.x - sx;
         c.ay = p.y - sy;
@@ -641,7 +703,7 @@ struct Group {
     }
 
     // Offers a batch of up to G candidates (lane-specific j when `has`).
-    __device__ void offer_batch(bool has, uint32_t j) {
+    __device__ __forceinline__ void offer_batch(bool has, uint32_t j) {
         Cand c;
         bool emptied = false;
         const bool live_c = has && prepare(j, c, emptied);
@@ -679,11 +741,14 @@ struct Group {
         }
     }
 
-    __device__ void init_box() {
-        const float s[3] = {sx, sy, sz};
+    __device__ __forceinline__ void init_box() {
         if (gl < 6) {
             const int axis = gl >> 1, side = gl & 1;
-            const float h = side ? 2.0f * (box_hi[axis] - s[axis]) : 2.0f * (s[axis] - box_lo[axis]);
+            // (a lane-dependent choice among the members sx, sy, sz would be
+            // compiled into a choice of addresses, moving them to scratch)
+            const float4 sp = pts[seed];
+            const float s = axis == 0 ? sp.x : (axis == 1 ? sp.y : sp.z);
+            const float h = side ? 2.0f * (hdr->box_hi[axis] - s) : 2.0f * (s - hdr->box_lo[axis]);
             float4 pl = make_float4(0.f, 0.f, 0.f, h);
             const float sg = side ? 1.0f : -1.0f;
             if (axis == 0) pl.x = sg;
@@ -698,6 +763,7 @@ struct Group {
             const int cx = gl >> 2 & 1, cy = gl >> 1 & 1, cz = gl & 1;
             const uint32_t X = cx, Y = 2 + cy, Z = 4 + cz;
             const int sgn = (cx ? 1 : -1) * (cy ? 1 : -1) * (cz ? 1 : -1);
+            // box corners: D = +-1 exactly, always resolved in FP32
             if (sgn > 0) make_vertex(gl, X, Y, Z);
             else make_vertex(gl, X, Z, Y);
         }
@@ -709,7 +775,7 @@ struct Group {
     }
 
     // ---------------------------------------------------------- traversal
-    __device__ void offer_leaf(uint32_t ref) {
+    __device__ __forceinline__ void offer_leaf(uint32_t ref) {
         if (gl == 0) c_leaves++;
         const uint32_t f = leaf_first(ref), c = leaf_count(ref);
         for (uint32_t o = 0; o < c && status == kSuccess && !overflow; o += G) {
@@ -719,7 +785,7 @@ struct Group {
     }
 
     // Smallest-key entry of the wave's wait list (wave arg-min), -1 if empty.
-    __device__ int list_argmin(int sz, float &kmin) const {
+    __device__ __forceinline__ int list_argmin(int sz, float &kmin) const {
         float k = FLT_MAX;
         int idx = -1;
 #pragma unroll
@@ -747,7 +813,7 @@ struct Group {
 
     // Wave-wide: does any cell of the wave still need this box?  key: the
     // smallest bound over the cells that need it.
-    __device__ bool packet_needs(float4 lo, float4 hi, float &key) const {
+    __device__ __forceinline__ bool packet_needs(float4 lo, float4 hi, float &key) const {
         float k = FLT_MAX;
         const bool need = live() && !node_prunable(lo, hi, k);
         const bool any = __ballot(gl == 0 && need) != 0ull;
@@ -766,7 +832,7 @@ struct Group {
     // (the order voronoi_bvh.cuh:246-283 keeps with its min-extraction pops),
     // re-testing a waiting box when it comes out.  A full list sends the
     // wave's cells to the next tier instead of dropping entries.
-    __device__ void traverse() {
+    __device__ __forceinline__ void traverse() {
         uint32_t ref = root_ref;
         int sz = 0;
         while (true) {
@@ -838,7 +904,7 @@ struct Group {
     // ---------------------------------------------------------- output
     // Row = sorted original ids of the owners of referenced planes
     // (voronoi_ultra.cu:125-138).  Returns false when the row does not fit.
-    __device__ bool emit(const CellParams &prm, uint32_t orig, int32_t *row_out, int row_cap) {
+    __device__ __forceinline__ bool emit(const CellParams &prm, uint32_t orig, int32_t *row_out, int row_cap) {
         int F = 0;
         if (status != kEmptyCell) {
             uint64_t used[kPW];
@@ -908,7 +974,7 @@ struct Group {
         return true;
     }
 
-    __device__ void start(uint32_t cell, const CellParams &prm) {
+    __device__ __forceinline__ void start(uint32_t cell, const CellParams &prm) {
         seed = cell;
         const float4 s = pts[cell];
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
@@ -944,7 +1010,7 @@ struct Group {
         in_guess = false;
     }
 
-    __device__ void setup(const CellParams &prm, GroupLDS<V, P> *s, WaveList<L> *w) {
+    __device__ __forceinline__ void setup(const CellParams &prm, GroupLDS<V, P> *s, WaveList<L> *w) {
         const int lane = threadIdx.x & (kWave - 1);
         const int grp = lane / G;
         S = s;
@@ -957,14 +1023,15 @@ struct Group {
         inv = prm.inv;
         nodes = prm.nodes;
         root_ref = prm.hdr->root_ref;
-        for (int k = 0; k < 3; ++k) box_lo[k] = prm.hdr->box_lo[k], box_hi[k] = prm.hdr->box_hi[k];
+        hdr = prm.hdr;
         c_recheck = c_unresolved = c_inconsistent = c_overflow = c_clips = c_cands = 0;
         c_nodes = c_leaves = c_skipped = 0;
     }
 
-    __device__ void flush(const CellParams &prm, uint32_t cells) const {
+    __device__ __forceinline__ void flush(const CellParams &prm, uint32_t cells) const {
         const uint32_t vals[12] = {c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands,
                                    c_nodes, cells, 0, 0, c_leaves, c_skipped};
+#pragma unroll
         for (int k = 0; k < 12; ++k) {
             const uint32_t s = wave_sum(vals[k]);
             if (lane64 == 0 && s) atomicAdd(&prm.stats[k], (unsigned long long)s);
