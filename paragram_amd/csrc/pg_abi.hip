@@ -76,7 +76,7 @@ int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes) {
     if (max_cells < 0) return fail(1, "max_cells must be non-negative");
     if (!bytes) return fail(1, "bytes is null");
     *bytes = pg::cell_scratch_bytes((waves + 3) / 4 * 4, pg::kTier1Waves, pg::tier1_capacity(max_cells),
-                                    pg::tier2_capacity(max_cells));
+                                    pg::tier2_capacity(max_cells), max_cells);
     return 0;
 }
 
@@ -102,7 +102,7 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
     const int w0 = (waves + 3) / 4 * 4;
     const int64_t big_cap = pg::tier1_capacity(end - begin);
     const int64_t huge_cap = pg::tier2_capacity(end - begin);
-    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap))
+    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap, end - begin))
         return fail(1, "cell scratch too small");
     pg::CellArgs a;
     a.n = n;

@@ -58,11 +58,13 @@ struct CellParams {
     int32_t *status;
     unsigned long long *stats;
     uint32_t *counters;           // [0] tier-0 work, [1] tier-1 queue length, [2] tier-1 work,
-                                  // [3] tier-2 queue length, [4] tier-2 work
+                                  // [3] tier-2 queue length, [4] tier-2 work,
+                                  // [5] cells flagged for FP64 tier 0, [6] FP64 tier-0 windows
     uint32_t *big_list;           // tier-1 queue of sorted cell ids
     uint32_t big_cap;
     uint32_t *huge_list;          // tier-2 queue of sorted cell ids
     uint32_t huge_cap;
+    uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
     uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
 };
 
@@ -98,11 +100,11 @@ __global__ void cell_prep_kernel(uint32_t *counters) {
 }
 
 // Every tier keeps its cells in LDS: the scratch holds the work counters
-// and the tier-1 and tier-2 queues.
-uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap) {
+// and the queues (FP64 tier 0: every cell of the range; tiers 1 and 2).
+uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells) {
     (void)waves;
     (void)waves1;
-    return 256 + 4 * (uint64_t)big_cap + 4 * (uint64_t)huge_cap;
+    return 256 + 4 * (uint64_t)big_cap + 4 * (uint64_t)huge_cap + 4 * (uint64_t)cells;
 }
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
@@ -137,15 +139,20 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.big_cap = (uint32_t)a.big_cap;
     prm.huge_list = (uint32_t *)(sc + 256 + 4 * a.big_cap);
     prm.huge_cap = (uint32_t)a.huge_cap;
+    prm.exact_list = (uint32_t *)(sc + 256 + 4 * a.big_cap + 4 * a.huge_cap);
     const int64_t count = a.end - a.begin;
     if (count <= 0) return hipSuccess;
-    if (cell_scratch_bytes(a.waves, a.waves1, a.big_cap, a.huge_cap) > a.scratch_bytes) return hipErrorInvalidValue;
+    if (cell_scratch_bytes(a.waves, a.waves1, a.big_cap, a.huge_cap, count) > a.scratch_bytes)
+        return hipErrorInvalidValue;
     const int64_t cells_per_block = kT0Threads / kT0Lanes;
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;
     const int64_t blocks = max_blocks < blocks_needed ? max_blocks : blocks_needed;
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
+    if (hipError_t e = fill_u32(prm.exact_list, count, 0u, stream)) return e;
     hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream, prm);
+    hipLaunchKernelGGL(cell_group_exact_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream,
+                       prm);
     hipLaunchKernelGGL(cell_group_big_kernel, dim3((unsigned)(a.waves1 * kWave / kT1Threads)), dim3(kT1Threads), 0,
                        stream, prm);
     hipLaunchKernelGGL(cell_group_huge_kernel, dim3((unsigned)kTier2Waves), dim3(kT2Threads), 0, stream, prm);

@@ -33,13 +33,6 @@
 
 namespace pg {
 
-// Best-first wait list shared by the cells of one wave (packet traversal).
-template <int L>
-struct alignas(16) WaveList {
-    uint32_t ref[L];  // parent << 1 | child
-    float key[L];
-};
-
 template <int V, int P>
 struct alignas(16) GroupLDS {
     float4 vpos[V];
@@ -47,6 +40,14 @@ struct alignas(16) GroupLDS {
     float2 plx[P];
     uint32_t vtri[V];
     uint32_t tmp[V];
+    uint32_t ctr[12];  // diagnostics, in the order of the statistics slots
+};
+
+// Candidates of one batch, staged for the group: (a, H) and (pH, owner).
+template <int G>
+struct alignas(16) CandLDS {
+    float4 aH[G];
+    float2 po[G];
 };
 
 struct Cand {
@@ -224,8 +225,71 @@ __device__ __noinline__ float4 fp64_position(const float2 *plx, const float4 *pt
     return make_float4(x[0], x[1], x[2], eps + kTiny32);
 }
 
+// ------------------------------------------------------------------ DPP
+// Row (16-lane) reductions and scans on the data-parallel-primitive path of
+// the VALU: no LDS round trip, unlike ds_bpermute shuffles.
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2, 3, 0, 1]
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;  // row_shr:n
+
+template <int C>
+__device__ __forceinline__ int dpp(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, C, 0xF, 0xF, true);  // lanes shifted in from outside the row read 0
+}
+template <int C>
+__device__ __forceinline__ float dppf(float x) {
+    return __int_as_float(dpp<C>(__float_as_int(x)));
+}
+__device__ __forceinline__ float row_min(float x) {
+    x = fminf(x, dppf<kDppXor1>(x));
+    x = fminf(x, dppf<kDppXor2>(x));
+    x = fminf(x, dppf<kDppRor4>(x));
+    return fminf(x, dppf<kDppRor8>(x));
+}
+__device__ __forceinline__ float row_max(float x) {
+    x = fmaxf(x, dppf<kDppXor1>(x));
+    x = fmaxf(x, dppf<kDppXor2>(x));
+    x = fmaxf(x, dppf<kDppRor4>(x));
+    return fmaxf(x, dppf<kDppRor8>(x));
+}
+__device__ __forceinline__ int row_sum(int x) {
+    x += dpp<kDppXor1>(x);
+    x += dpp<kDppXor2>(x);
+    x += dpp<kDppRor4>(x);
+    return x + dpp<kDppRor8>(x);
+}
+__device__ __forceinline__ uint32_t row_or(uint32_t x) {
+    x |= (uint32_t)dpp<kDppXor1>((int)x);
+    x |= (uint32_t)dpp<kDppXor2>((int)x);
+    x |= (uint32_t)dpp<kDppRor4>((int)x);
+    return x | (uint32_t)dpp<kDppRor8>((int)x);
+}
+// inclusive prefix sum within the row
+__device__ __forceinline__ int row_scan(int x) {
+    x += dpp<kDppShr1>(x);
+    x += dpp<kDppShr2>(x);
+    x += dpp<kDppShr4>(x);
+    return x + dpp<kDppShr8>(x);
+}
+__device__ __forceinline__ int rl(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+__device__ __forceinline__ uint32_t rl(uint32_t x, int lane) { return (uint32_t)__builtin_amdgcn_readlane((int)x, lane); }
+__device__ __forceinline__ float rl(float x, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// Values the whole wave agrees on, marked as such for the compiler (they
+// then live in scalar registers).
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+
 // ------------------------------------------------------------------ group
-template <int G, int V, int P, int L>
+// kExact: the tier carries the FP64 recheck path.  A tier without it gives
+// up on a cell at the first test its FP32 filter cannot decide (`bail`), and
+// the cell is rebuilt by the FP64-capable twin of the tier.
+template <int G, int V, int P, int L, bool kExact>
 struct Group {
     static constexpr int kRows = V / G;               // vertex slots per lane
     static constexpr int kW = V / 64;                 // words of a vertex mask
@@ -234,10 +298,14 @@ struct Group {
     static constexpr uint64_t kRowMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     static constexpr int kB = P < 256 ? 8 : 10;       // bits of a plane slot in a triple
     static constexpr uint32_t kBMask = (1u << kB) - 1u;
-    static_assert(64 % G == 0 && V % 64 == 0 && P < (1 << kB) && 3 * kRows <= 64, "group geometry");
+    static_assert((G == 16 || G == 64) && L % kWave == 0 && V % 64 == 0 && P < (1 << kB) && 3 * kRows <= 64, "group geometry");
 
+    static constexpr bool kIsExact = kExact;
+    static constexpr int kLS = L / kWave;  // wait-list slots per lane
     GroupLDS<V, P> *S;
-    WaveList<L> *W;
+    CandLDS<G> *C;
+    float lk[kLS];
+    uint32_t lr[kLS];
     int gl;      // lane within the group
     int gshift;  // bit offset of the group in a wave ballot
     int lane64;
@@ -253,52 +321,61 @@ struct Group {
     float sx, sy, sz, sw;
     uint64_t valid[kW];
     int np;
-    float L_[3], U_[3], L2[3], U2[3];  // AABB of the cell and its squared ends
+    float L_[3], U_[3];  // AABB of the cell
     bool bounded;
     int status;
-    bool overflow;
+    bool overflow;  // the cell left this tier (capacity, or `bail`)
+    bool bail;      // an FP32 test was undecided and the tier has no FP64 path
     bool active;  // the group holds a cell in this round
     bool dirty;   // the cell changed since the AABB was computed
     uint64_t bloom;
     bool in_guess;
 
-    uint32_t c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands, c_nodes, c_leaves, c_skipped;
+    // diagnostics counters live in LDS (registers are the scarce resource)
+    __device__ __forceinline__ void count(int slot, uint32_t v = 1u) const { atomicAdd(&S->ctr[slot], v); }
 
     // ---------------------------------------------------------- group ops
     __device__ __forceinline__ uint64_t gballot(bool p) const { return (__ballot(p) >> gshift) & kRowMask; }
     __device__ __forceinline__ bool gany(bool p) const { return gballot(p) != 0ull; }
     __device__ __forceinline__ bool live() const { return active && status == kSuccess && !overflow; }
-    __device__ __forceinline__ float gbf(float x, int k) const { return __shfl(x, k, G); }
     __device__ __forceinline__ int gbi(int x, int k) const { return __shfl(x, k, G); }
+    // Reductions and scans: DPP within each 16-lane row, rows combined
+    // through readlane when the group is the whole wave.
     __device__ __forceinline__ uint64_t gor64(uint64_t x) const {
-        uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-        for (int o = G / 2; o > 0; o >>= 1) {
-            lo |= (uint32_t)__shfl_xor((int)lo, o, G);
-            hi |= (uint32_t)__shfl_xor((int)hi, o, G);
+        uint32_t lo = row_or((uint32_t)x), hi = row_or((uint32_t)(x >> 32));
+        if constexpr (G == 64) {
+            lo = rl(lo, 0) | rl(lo, 16) | rl(lo, 32) | rl(lo, 48);
+            hi = rl(hi, 0) | rl(hi, 16) | rl(hi, 32) | rl(hi, 48);
         }
         return ((uint64_t)hi << 32) | lo;
     }
     __device__ __forceinline__ float gmin(float x) const {
-        for (int o = G / 2; o > 0; o >>= 1) x = fminf(x, __shfl_xor(x, o, G));
+        x = row_min(x);
+        if constexpr (G == 64) x = fminf(fminf(rl(x, 0), rl(x, 16)), fminf(rl(x, 32), rl(x, 48)));
         return x;
     }
     __device__ __forceinline__ float gmax(float x) const {
-        for (int o = G / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, G));
+        x = row_max(x);
+        if constexpr (G == 64) x = fmaxf(fmaxf(rl(x, 0), rl(x, 16)), fmaxf(rl(x, 32), rl(x, 48)));
         return x;
     }
     __device__ __forceinline__ int gsum(int x) const {
-        for (int o = G / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, G);
+        x = row_sum(x);
+        if constexpr (G == 64) x = rl(x, 0) + rl(x, 16) + rl(x, 32) + rl(x, 48);
         return x;
     }
     // exclusive prefix sum over the group; total in `tot`
     __device__ __forceinline__ int gexcl(int x, int &tot) const {
-        int s = x;
-        for (int o = 1; o < G; o <<= 1) {
-            const int y = __shfl_up(s, o, G);
-            if (gl >= o) s += y;
+        const int incl = row_scan(x);
+        if constexpr (G == 16) {
+            tot = row_sum(x);
+            return incl - x;
+        } else {
+            const int t0 = rl(incl, 15), t1 = rl(incl, 31), t2 = rl(incl, 47), t3 = rl(incl, 63);
+            const int r = lane64 >> 4;
+            tot = t0 + t1 + t2 + t3;
+            return incl - x + (r > 0 ? t0 : 0) + (r > 1 ? t1 : 0) + (r > 2 ? t2 : 0);
         }
-        tot = __shfl(s, G - 1, G);
-        return s - x;
     }
     // row k of a vertex mask: the bits of slots gl + G k, lane order
     __device__ __forceinline__ uint64_t row(const uint64_t (&m)[kW], int k) const {
@@ -312,10 +389,10 @@ struct Group {
     // (free functions below: a call that passed `this` would move the whole
     // group state from registers to scratch memory)
     __device__ __forceinline__ bool recheck64(int v, int q_owner) {
-        c_recheck++;
+        count(kStatFp64Rechecks);
         const int r = fp64_recheck<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw, q_owner);
-        if (r & 2) c_unresolved++;
-        if (r & 4) c_inconsistent++;
+        if (r & 2) count(kStatUnresolved);
+        if (r & 4) count(kStatInconsistent);
         return (r & 1) != 0;
     }
 
@@ -358,7 +435,7 @@ struct Group {
             const float ez = (kG32 * pNz + fabsf(z) * eD) * den + fabsf(z) * kSlack;
             pos = make_float4(x, y, z, fmaxf(ex, fmaxf(ey, ez)) + kTiny32);
         } else if (D < -eD && pD > kTiny32) {
-            c_inconsistent++;
+            count(kStatInconsistent);
         }
         S->vpos[slot] = pos;
         S->vtri[slot] = tri;
@@ -411,7 +488,6 @@ struct Group {
         bounded = !gany(!ok);
         L_[0] = gmin(lo0), L_[1] = gmin(lo1), L_[2] = gmin(lo2);
         U_[0] = gmax(hi0), U_[1] = gmax(hi1), U_[2] = gmax(hi2);
-        for (int k = 0; k < 3; ++k) L2[k] = L_[k] * L_[k], U2[k] = U_[k] * U_[k];
     }
 
     // Can the plane of candidate c not cut the cell?  max over the AABB of
@@ -448,9 +524,9 @@ struct Group {
         if (!bounded) return false;
         // the octants a box overlaps form a product set, so the largest
         // corner radius over them is a sum of per-axis maxima
-        const float rho2 = (fmaxf(alx <= 0.0f ? L2[0] : 0.0f, ahx >= 0.0f ? U2[0] : 0.0f) +
-                            fmaxf(aly <= 0.0f ? L2[1] : 0.0f, ahy >= 0.0f ? U2[1] : 0.0f) +
-                            fmaxf(alz <= 0.0f ? L2[2] : 0.0f, ahz >= 0.0f ? U2[2] : 0.0f)) *
+        const float rho2 = (fmaxf(alx <= 0.0f ? L_[0] * L_[0] : 0.0f, ahx >= 0.0f ? U_[0] * U_[0] : 0.0f) +
+                            fmaxf(aly <= 0.0f ? L_[1] * L_[1] : 0.0f, ahy >= 0.0f ? U_[1] * U_[1] : 0.0f) +
+                            fmaxf(alz <= 0.0f ? L_[2] * L_[2] : 0.0f, ahz >= 0.0f ? U_[2] * U_[2] : 0.0f)) *
                            (1.0f + kSlack);
         const float d2lo = d2 * (1.0f - kSlack);
         const float h = d2lo - dw - kSlack * (fabsf(lo.w) + fabsf(sw)) - kTiny32;
@@ -555,10 +631,15 @@ struct Group {
             if (b > 0) mine |= 1u << k;
             if (b < 0) unsure |= 1u << k;
         }
-        while (unsure) {  // one FP64 call site
-            const int k = (int)__builtin_ctz(unsure);
-            unsure &= unsure - 1u;
-            if (recheck64(gl + G * k, q.owner)) mine |= 1u << k;
+        if constexpr (kExact) {
+            while (unsure) {  // one FP64 call site
+                const int k = (int)__builtin_ctz(unsure);
+                unsure &= unsure - 1u;
+                if (recheck64(gl + G * k, q.owner)) mine |= 1u << k;
+            }
+        } else if (gany(unsure != 0u)) {
+            overflow = bail = true;
+            return false;
         }
 #pragma unroll
         for (int k = 0; k < kRows; ++k) rem[(G * k) >> 6] |= gballot((mine >> k) & 1u) << ((G * k) & 63);
@@ -600,11 +681,18 @@ struct Group {
             }
             const bool has_tri = gl < R;
             tr = has_tri ? S->tmp[gl] : 0u;
+            const uint32_t t0 = tri_get<kB>(tr, 0), t1 = tri_get<kB>(tr, 1), t2 = tri_get<kB>(tr, 2);
+            bool tw0 = false, tw1 = false, tw2 = false;
+            for (int u = 0; u < R; ++u) {  // broadcast reads of the removed triangles
+                const uint32_t o = S->tmp[u];
+                tw0 |= tri_has_edge<kB>(o, t1, t0);
+                tw1 |= tri_has_edge<kB>(o, t2, t1);
+                tw2 |= tri_has_edge<kB>(o, t0, t2);
+            }
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-                const uint32_t x = tri_get<kB>(tr, e), y = tri_get<kB>(tr, e == 2 ? 0 : e + 1);
-                bool twin = false;
-                for (int u = 0; u < R; ++u) twin |= tri_has_edge<kB>((uint32_t)__shfl((int)tr, u, G), y, x);
+                const uint32_t x = e == 0 ? t0 : (e == 1 ? t1 : t2), y = e == 0 ? t1 : (e == 1 ? t2 : t0);
+                const bool twin = e == 0 ? tw0 : (e == 1 ? tw1 : tw2);
                 if (has_tri && !twin) {
                     emask |= 1ull << e;
                     sx_ += (int)(x * x + 1u);
@@ -619,7 +707,7 @@ struct Group {
         // every plane of the boundary cycle starts one edge and ends one
         if (nb < 3 || gsum(sx_) != gsum(sy_)) {
             status = kInconsistentBoundary;
-            if (gl == 0) c_inconsistent++;
+            if (gl == 0) count(kStatInconsistent);
             return true;
         }
         uint64_t freem[kW];
@@ -647,7 +735,14 @@ struct Group {
         }
         uint64_t added[kW];
         mask_first(freem, nb, added);
-        if (gany(bad)) fix_vertices(added);
+        if (gany(bad)) {
+            if constexpr (kExact) {
+                fix_vertices(added);
+            } else {
+                overflow = bail = true;
+                return false;
+            }
+        }
 #pragma unroll
         for (int w = 0; w < kW; ++w) valid[w] = (valid[w] & ~rem[w]) | added[w];
         dirty = true;
@@ -713,23 +808,30 @@ struct Group {
             for (int w = 0; w < kW; ++w) valid[w] = 0;
             return;
         }
-        if (gl == 0) c_skipped += __popcll(gballot(has && !live_c));
+        const uint64_t skipped = gballot(has && !live_c);
+        if (gl == 0 && skipped) count(kStatSkipped, (uint32_t)__popcll(skipped));
         uint64_t todo = gballot(live_c);
+        if (live_c) {
+            C->aH[gl] = make_float4(c.ax, c.ay, c.az, c.H);
+            C->po[gl] = make_float2(c.pH, __int_as_float(c.owner));
+        }
         while (todo) {
             const int b = (int)__builtin_ctzll(todo);
             todo &= todo - 1ull;
             Cand q;
-            q.ax = gbf(c.ax, b);
-            q.ay = gbf(c.ay, b);
-            q.az = gbf(c.az, b);
-            q.H = gbf(c.H, b);
-            q.pH = gbf(c.pH, b);
-            q.pa1 = gbf(c.pa1, b);
-            q.owner = gbi(c.owner, b);
+            const float4 aH = C->aH[b];
+            const float2 po = C->po[b];
+            q.ax = aH.x;
+            q.ay = aH.y;
+            q.az = aH.z;
+            q.H = aH.w;
+            q.pH = po.x;
+            q.pa1 = fabsf(q.ax) + fabsf(q.ay) + fabsf(q.az);
+            q.owner = __float_as_int(po.y);
             if (bloom && has_plane(q.owner)) continue;
-            if (gl == 0) c_cands++;
+            if (gl == 0) count(kStatCandidates);
             if (clip(q)) {
-                if (gl == 0) c_clips++;
+                if (gl == 0) count(kStatClips);
                 if (status != kSuccess) return;
             } else if (overflow) {
                 return;
@@ -776,7 +878,7 @@ struct Group {
 
     // ---------------------------------------------------------- traversal
     __device__ __forceinline__ void offer_leaf(uint32_t ref) {
-        if (gl == 0) c_leaves++;
+        if (gl == 0) count(kStatLeaves);
         const uint32_t f = leaf_first(ref), c = leaf_count(ref);
         for (uint32_t o = 0; o < c && status == kSuccess && !overflow; o += G) {
             const uint32_t j = f + o + (uint32_t)gl;
@@ -784,31 +886,60 @@ struct Group {
         }
     }
 
-    // Smallest-key entry of the wave's wait list (wave arg-min), -1 if empty.
+    // ---------------------------------------------------------- wait list
+    // Best-first wait list shared by the cells of one wave, in registers:
+    // entry t sits in slot t / 64 of lane t % 64.  Entries are
+    // parent << 1 | child, keyed by the child's bound.
+    __device__ __forceinline__ void list_put(int t, uint32_t e, float key) {
+        const bool me = lane64 == (t & (kWave - 1));
+        const int sl = t / kWave;
+#pragma unroll
+        for (int j = 0; j < kLS; ++j) {
+            const bool w = me && j == sl;
+            lr[j] = w ? e : lr[j];
+            lk[j] = w ? key : lk[j];
+        }
+    }
+
+    // Smallest-key entry (wave arg-min), -1 if the list is empty.
     __device__ __forceinline__ int list_argmin(int sz, float &kmin) const {
+        kmin = FLT_MAX;
+        if (sz == 0) return -1;
         float k = FLT_MAX;
         int idx = -1;
 #pragma unroll
-        for (int j = 0; j < L / kWave; ++j) {
+        for (int j = 0; j < kLS; ++j) {
             const int t = lane64 + kWave * j;
-            if (t < sz) {
-                const float kt = W->key[t];
-                if (idx < 0 || kt < k) {
-                    k = kt;
-                    idx = t;
-                }
+            if (t < sz && (idx < 0 || lk[j] < k)) {
+                k = lk[j];
+                idx = t;
             }
         }
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const float ok = __shfl_xor(k, o);
-            const int oi = __shfl_xor(idx, o);
-            if (oi >= 0 && (idx < 0 || ok < k || (ok == k && oi < idx))) {
-                k = ok;
-                idx = oi;
-            }
+        const float r = row_min(k);
+        const float m = uniform(fminf(fminf(rl(r, 0), rl(r, 16)), fminf(rl(r, 32), rl(r, 48))));
+        const uint64_t hit = __ballot(idx >= 0 && k == m);
+        kmin = m;
+        if (!hit) return sz - 1;  // unordered keys: take the last entry
+        return rl(idx, (int)__builtin_ctzll(hit));
+    }
+
+    // Removes entry im (the last entry moves into its place); returns it.
+    __device__ __forceinline__ uint32_t list_take(int &sz, int im) {
+        const int last = sz - 1;
+        uint32_t re = 0u, rlast = 0u;
+        float klast = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kLS; ++j) {
+            re = j == im / kWave ? lr[j] : re;
+            rlast = j == last / kWave ? lr[j] : rlast;
+            klast = j == last / kWave ? lk[j] : klast;
         }
-        kmin = k;
-        return idx;
+        const uint32_t e = rl(re, im & (kWave - 1));
+        const uint32_t el = rl(rlast, last & (kWave - 1));
+        const float kl = rl(klast, last & (kWave - 1));
+        list_put(im, el, kl);
+        sz = last;
+        return e;
     }
 
     // Wave-wide: does any cell of the wave still need this box?  key: the
@@ -818,8 +949,8 @@ struct Group {
         const bool need = live() && !node_prunable(lo, hi, k);
         const bool any = __ballot(gl == 0 && need) != 0ull;
         float kk = need ? k : FLT_MAX;
-        for (int o = G; o < kWave; o <<= 1) kk = fminf(kk, __shfl_xor(kk, o));
-        key = kk;
+        if constexpr (G == 16) kk = fminf(fminf(rl(kk, 0), rl(kk, 16)), fminf(rl(kk, 32), rl(kk, 48)));
+        key = uniform(kk);
         return any;
     }
 
@@ -833,19 +964,19 @@ struct Group {
     // re-testing a waiting box when it comes out.  A full list sends the
     // wave's cells to the next tier instead of dropping entries.
     __device__ __forceinline__ void traverse() {
-        uint32_t ref = root_ref;
+        uint32_t ref = uniform(root_ref);
         int sz = 0;
         while (true) {
             bool have = false;
             if (is_leaf(ref)) {
                 if (live()) offer_leaf(ref);
             } else {
-                if (lane64 == 0) c_nodes++;
+                if (lane64 == 0) count(kStatNodes);
                 const Node nd = nodes[ref];
                 float k0, k1;
                 const bool n0 = packet_needs(nd.lo0, nd.hi0, k0);
                 const bool n1 = packet_needs(nd.lo1, nd.hi1, k1);
-                const uint32_t r0 = __float_as_uint(nd.hi0.w), r1 = __float_as_uint(nd.hi1.w);
+                const uint32_t r0 = uniform(__float_as_uint(nd.hi0.w)), r1 = uniform(__float_as_uint(nd.hi1.w));
                 if (n0 || n1) {
                     const bool near0 = !n1 || (n0 && k0 <= k1);
                     const float knear = near0 ? k0 : k1;
@@ -854,11 +985,7 @@ struct Group {
                             overflow = true;
                             return;
                         }
-                        if (lane64 == 0) {
-                            W->ref[sz] = (ref << 1) | (near0 ? 1u : 0u);
-                            W->key[sz] = near0 ? k1 : k0;
-                        }
-                        sz++;
+                        list_put(sz++, (ref << 1) | (near0 ? 1u : 0u), near0 ? k1 : k0);
                     }
                     float kmin;
                     const int im = list_argmin(sz, kmin);
@@ -870,11 +997,7 @@ struct Group {
                             overflow = true;
                             return;
                         }
-                        if (lane64 == 0) {
-                            W->ref[sz] = (ref << 1) | (near0 ? 0u : 1u);
-                            W->key[sz] = knear;
-                        }
-                        sz++;
+                        list_put(sz++, (ref << 1) | (near0 ? 0u : 1u), knear);
                     }
                 }
             }
@@ -882,19 +1005,14 @@ struct Group {
                 float kmin;
                 const int im = list_argmin(sz, kmin);
                 if (im < 0) return;
-                const uint32_t e = W->ref[im];
-                if (lane64 == 0) {
-                    W->ref[im] = W->ref[sz - 1];
-                    W->key[im] = W->key[sz - 1];
-                }
-                sz--;
+                const uint32_t e = list_take(sz, im);
                 const uint32_t parent = e >> 1, c = e & 1u;
                 const Node nd = nodes[parent];
                 float k;
                 const float4 lo = c ? nd.lo1 : nd.lo0;
                 const float4 hi = c ? nd.hi1 : nd.hi0;
                 if (packet_needs(lo, hi, k)) {
-                    ref = __float_as_uint(hi.w);
+                    ref = uniform(__float_as_uint(hi.w));
                     have = true;
                 }
             }
@@ -980,6 +1098,7 @@ struct Group {
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
         status = kSuccess;
         overflow = false;
+        bail = false;
         init_box();
         const uint32_t orig = perm[cell];
         in_guess = true;
@@ -1010,11 +1129,13 @@ struct Group {
         in_guess = false;
     }
 
-    __device__ __forceinline__ void setup(const CellParams &prm, GroupLDS<V, P> *s, WaveList<L> *w) {
+    __device__ __forceinline__ void setup(const CellParams &prm, GroupLDS<V, P> *s, CandLDS<G> *c) {
         const int lane = threadIdx.x & (kWave - 1);
         const int grp = lane / G;
         S = s;
-        W = w;
+        C = c;
+#pragma unroll
+        for (int j = 0; j < kLS; ++j) lk[j] = 0.0f, lr[j] = 0u;
         gl = lane & (G - 1);
         lane64 = lane;
         gshift = grp * G;
@@ -1024,13 +1145,14 @@ struct Group {
         nodes = prm.nodes;
         root_ref = prm.hdr->root_ref;
         hdr = prm.hdr;
-        c_recheck = c_unresolved = c_inconsistent = c_overflow = c_clips = c_cands = 0;
-        c_nodes = c_leaves = c_skipped = 0;
+        if (gl < 12) S->ctr[gl] = 0u;
     }
 
     __device__ __forceinline__ void flush(const CellParams &prm, uint32_t cells) const {
-        const uint32_t vals[12] = {c_recheck, c_unresolved, c_inconsistent, c_overflow, c_clips, c_cands,
-                                   c_nodes, cells, 0, 0, c_leaves, c_skipped};
+        uint32_t vals[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) vals[k] = gl == 0 ? S->ctr[k] : 0u;
+        vals[kStatCells] = cells;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             const uint32_t s = wave_sum(vals[k]);
@@ -1044,66 +1166,100 @@ constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Th
 constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 256, kT1Threads = 128;
 constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 1024, kT2Threads = 64;
 static_assert(kT0Planes <= kRowCap && kT1Planes <= kTier1Row && kT2Planes <= kTier2Row, "rows fit their strides");
-using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List>;
-using Tier1 = Group<kT1Lanes, kT1Verts, kT1Planes, kT1List>;
-using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List>;
+using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, false>;
+using Tier0x = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, true>;
+using Tier1 = Group<kT1Lanes, kT1Verts, kT1Planes, kT1List, true>;
+using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 
-// Tier 0: a wave holds four Morton-consecutive cells per round (pulled from
-// a global counter) and traverses for them together.  A cell that outgrows
-// the tier is queued for tier 1.
-__global__ void __launch_bounds__(kT0Threads) cell_group_kernel(const CellParams prm) {
+// Tier 0: a wave holds four cells per round and traverses for them
+// together.  The FP32-only instance takes Morton-consecutive cells of the
+// range from a global counter; a cell it gives up on (`bail`) is flagged for
+// the FP64-capable instance (kExact), whose waves scan the flags in windows
+// of 64 Morton-consecutive cells and take the flagged cells four at a time
+// (neighbours again, so the shared traversal stays coherent).  A cell that
+// outgrows the tier is queued for tier 1.
+template <typename T>
+__device__ __forceinline__ void tier0_body(const CellParams &prm) {
+    constexpr bool kExactTier = T::kIsExact;
     constexpr int kCellsPerWave = kWave / kT0Lanes;
     __shared__ GroupLDS<kT0Verts, kT0Planes> lds[kT0Threads / kT0Lanes];
-    __shared__ WaveList<kT0List> wlist[kT0Threads / kWave];
+    __shared__ CandLDS<kT0Lanes> cands[kT0Threads / kT0Lanes];
     const int lane = threadIdx.x & (kWave - 1);
     const int grp = lane / kT0Lanes;
-    Tier0 g;
-    g.setup(prm, &lds[(threadIdx.x / kWave) * kCellsPerWave + grp], &wlist[threadIdx.x / kWave]);
+    if (kExactTier && prm.counters[5] == 0u) return;  // no cell was flagged
+    T g;
+    g.setup(prm, &lds[(threadIdx.x / kWave) * kCellsPerWave + grp], &cands[(threadIdx.x / kWave) * kCellsPerWave + grp]);
     uint32_t cells = 0;
     const int64_t count = prm.end - prm.begin;
-    while (true) {
-        uint32_t c0 = 0;
-        if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kCellsPerWave);
-        c0 = (uint32_t)__shfl((int)c0, 0);
-        if ((int64_t)c0 >= count) break;
+    // one round: cell begin + c for an active group
+    auto round = [&](bool active, int64_t c) {
         const uint64_t t0 = clock64();
-        const int64_t c = (int64_t)c0 + grp;
-        g.active = c < count;
-        const uint32_t cell = (uint32_t)(prm.begin + (g.active ? c : 0));
-        if (g.active) g.start(cell, prm);
+        g.active = active;
+        const uint32_t cell = (uint32_t)(prm.begin + (active ? c : 0));
+        if (active) g.start(cell, prm);
         g.traverse();
-        if (!g.active) continue;
+        if (!active) return;
         const uint32_t orig = prm.perm[cell];
-        const int64_t local = (int64_t)cell - prm.begin;
         bool done = false;
         if (!g.overflow) {
-            if (g.gl == 0) prm.row_offsets[cell] = local * prm.row_stride;
-            done = g.emit(prm, orig, prm.rows + local * prm.row_stride, prm.row_stride);
+            if (g.gl == 0) prm.row_offsets[cell] = c * prm.row_stride;
+            done = g.emit(prm, orig, prm.rows + c * prm.row_stride, prm.row_stride);
         }
-        if (!done && g.gl == 0) {  // queue for the large-capacity tier
-            g.c_overflow++;
-            const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
-            if (slot < prm.big_cap) {
-                prm.big_list[slot] = cell;
-            } else {
-                prm.row_offsets[cell] = local * prm.row_stride;
-                prm.counts[orig] = 0;
-                prm.status[orig] = kPlaneOverflow;
+        if (!done && g.gl == 0) {
+            if (!kExactTier && g.bail) {  // flag for the FP64-capable instance
+                prm.exact_list[c] = 1u;
+                atomicAdd(&prm.counters[5], 1u);
+            } else {  // queue for the large-capacity tier
+                g.count(kStatOverflow);
+                const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
+                if (slot < prm.big_cap) {
+                    prm.big_list[slot] = cell;
+                } else {
+                    prm.row_offsets[cell] = c * prm.row_stride;
+                    prm.counts[orig] = 0;
+                    prm.status[orig] = kPlaneOverflow;
+                }
             }
         }
         if (prm.cycles && g.gl == 0) prm.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
-        if (g.gl == 0) cells++;
+        if (g.gl == 0 && !g.bail) cells++;
+    };
+    if constexpr (!kExactTier) {
+        while (true) {
+            uint32_t c0 = 0;
+            if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kCellsPerWave);
+            c0 = uniform(c0);
+            if ((int64_t)c0 >= count) break;
+            round((int64_t)c0 + grp < count, (int64_t)c0 + grp);
+        }
+    } else {
+        while (true) {
+            uint32_t w = 0;
+            if (lane == 0) w = atomicAdd(&prm.counters[6], 1u);
+            const int64_t base = (int64_t)uniform(w) * kWave;
+            if (base >= count) break;
+            uint64_t bits = __ballot(base + lane < count && prm.exact_list[base + lane] != 0u);
+            while (bits) {
+                uint64_t b = bits;  // the grp-th flagged cell of the window
+                for (int t = 0; t < grp; ++t) b &= b - 1ull;
+                for (int t = 0; t < kCellsPerWave; ++t) bits &= bits - 1ull;
+                round(b != 0ull, base + (b ? (int64_t)__builtin_ctzll(b) : 0));
+            }
+        }
     }
     g.flush(prm, cells);
 }
 
+__global__ void __launch_bounds__(kT0Threads) cell_group_kernel(const CellParams prm) { tier0_body<Tier0>(prm); }
+__global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) { tier0_body<Tier0x>(prm); }
+
 // Tier 1: one cell per wave, the queued cells; rows after the tier-0 slab.
 __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT1Verts, kT1Planes> lds[kT1Threads / kWave];
-    __shared__ WaveList<kT1List> wlist[kT1Threads / kWave];
+    __shared__ CandLDS<kT1Lanes> cands[kT1Threads / kWave];
     const int lane = threadIdx.x & (kWave - 1);
     Tier1 g;
-    g.setup(prm, &lds[threadIdx.x / kWave], &wlist[threadIdx.x / kWave]);
+    g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
     const uint32_t queued = min(prm.counters[1], prm.big_cap);
     while (true) {
         uint32_t k = 0;
@@ -1120,7 +1276,7 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         if (lane == 0) prm.row_offsets[cell] = off;
         if (g.overflow || !g.emit(prm, orig, prm.rows + off, kTier1Row)) {
             if (lane == 0) {  // queue for tier 2
-                g.c_overflow++;
+                g.count(kStatOverflow);
                 const uint32_t slot = atomicAdd(&prm.counters[3], 1u);
                 if (slot < prm.huge_cap) {
                     prm.huge_list[slot] = cell;
@@ -1140,10 +1296,10 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
 // that outgrow tier 1; rows after the tier-1 rows.
 __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT2Verts, kT2Planes> lds;
-    __shared__ WaveList<kT2List> wlist;
+    __shared__ CandLDS<kT2Lanes> cands;
     const int lane = threadIdx.x & (kWave - 1);
     Tier2 g;
-    g.setup(prm, &lds, &wlist);
+    g.setup(prm, &lds, &cands);
     const uint32_t queued = min(prm.counters[3], prm.huge_cap);
     while (true) {
         uint32_t k = 0;
@@ -1163,7 +1319,7 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
             // beyond every tier: report the reference's capacity status with
             // an empty row (counted, never silent)
             if (lane == 0) {
-                g.c_overflow++;
+                g.count(kStatOverflow);
                 prm.counts[orig] = 0;
                 prm.status[orig] = kPlaneOverflow;
             }

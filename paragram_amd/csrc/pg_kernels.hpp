@@ -37,7 +37,7 @@ struct CellArgs {
     uint32_t *cycles;             // optional per-cell cost counters
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
-uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap);
+uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells);
 
 // CSR helpers.
 hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int64_t begin, int64_t end,
