@@ -537,7 +537,34 @@ struct Group {
         const float M = fmaxf(fabsf(alx), fabsf(ahx)) * fmaxf(fabsf(L_[0]), fabsf(U_[0])) +
                         fmaxf(fabsf(aly), fabsf(ahy)) * fmaxf(fabsf(L_[1]), fabsf(U_[1])) +
                         fmaxf(fabsf(alz), fabsf(ahz)) * fmaxf(fabsf(L_[2]), fabsf(U_[2]));
-        return (hlow - S_) > kSlack * (d2 + fabsf(lo.w) + fabsf(sw) + M) + kTiny32;
+        if ((hlow - S_) > kSlack * (d2 + fabsf(lo.w) + fabsf(sw) + M) + kTiny32) return true;
+        return balls_clear(alx, aly, alz, ahx, ahy, ahz, dw + kSlack * (fabsf(lo.w) + fabsf(sw)) + kTiny32);
+    }
+
+    // Exact form of the prune, per vertex (group op).  Vertex y is beyond the
+    // plane of p iff a . y - |a|^2 + dw > 0, i.e. |a - c|^2 < |c|^2 + dw with
+    // c = y / 2: p lies in the ball about the vertex through the seed (grown
+    // by the weight difference).  No point of the box cuts the cell when the
+    // box misses every vertex's ball.  Each vertex's error radius eps widens
+    // the test: |c_k - c~_k| <= eps / 2.
+    __device__ __forceinline__ bool balls_clear(float alx, float aly, float alz, float ahx, float ahy, float ahz,
+                                                float dwmax) const {
+        bool cut = false;
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            if (!own(valid, k)) continue;
+            const float4 p = S->vpos[gl + G * k];
+            const float e = 0.5f * p.w;
+            const float cx = 0.5f * p.x, cy = 0.5f * p.y, cz = 0.5f * p.z;
+            const float ex = fmaxf(fmaxf(alx - cx, cx - ahx) - e, 0.0f);
+            const float ey = fmaxf(fmaxf(aly - cy, cy - ahy) - e, 0.0f);
+            const float ez = fmaxf(fmaxf(alz - cz, cz - ahz) - e, 0.0f);
+            const float rx = fabsf(cx) + e, ry = fabsf(cy) + e, rz = fabsf(cz) + e;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            cut |= !(d2 * (1.0f - kSlack) > r2 * (1.0f + kSlack) + dwmax);
+        }
+        return !gany(cut);
     }
 
     // ---------------------------------------------------------- planes

@@ -266,55 +266,127 @@ __device__ __forceinline__ uint32_t load_coherent_u(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bottom-up refit: one thread per sorted point climbs; the second arrival at
-// a node (arrival counter) builds the node's traversal record from both
-// children and its AABB / max weight.
-__global__ void refit_kernel(const float4 *__restrict__ pts, int64_t n, const uint32_t *__restrict__ children,
-                             const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
-                             float4 *__restrict__ boxes, uint32_t *__restrict__ flags, Node *__restrict__ nodes,
-                             int leaf_size) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    uint32_t p = parents[(n - 1) + t];
-    // the hierarchy is at most 30 + 26 levels deep; the cap only bounds the loop
-    for (int level = 0; level < 128 && p != 0xFFFFFFFFu && (int64_t)p < n - 1; ++level) {
-        __threadfence();
-        const uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0) return;
-        __threadfence();
-        float4 lo[2], hi[2];
-        uint32_t ref[2];
-        for (int c = 0; c < 2; ++c) {
-            const uint32_t ch = children[2 * p + c];
-            uint32_t first, last;
-            if (ch & 0x80000000u) {
-                const uint32_t prim = ch & 0x7fffffffu;
-                const float4 q = pts[prim];
-                lo[c] = q;
-                hi[c] = make_float4(q.x, q.y, q.z, 0.0f);
-                first = last = prim;
-            } else {
-                const float *b = (const float *)&boxes[2 * (int64_t)ch];
-                lo[c] = make_float4(load_coherent(b + 0), load_coherent(b + 1), load_coherent(b + 2), load_coherent(b + 3));
-                hi[c] = make_float4(load_coherent(b + 4), load_coherent(b + 5), load_coherent(b + 6), 0.0f);
-                const uint2 r = ranges[ch];
-                first = r.x;
-                last = r.y;
-            }
-            const uint32_t count = last - first + 1;
-            ref[c] = (count <= (uint32_t)leaf_size) ? leaf_ref(first, count) : ch;
+// Bottom-up refit in two phases.  The hierarchy's caches are per XCD, so a
+// device-scope fence writes back and invalidates L2; the phases keep those
+// fences off the bulk of the nodes.
+//   1. One workgroup per chunk of kRefitChunk Morton-consecutive points: a
+//      thread per point climbs; the second arrival at a node whose range lies
+//      inside the chunk builds it (arrival counters and child boxes in LDS,
+//      workgroup-scope fences).  A Karras node's index is one end of its
+//      range, so the chunk's nodes have indices inside the chunk.
+//   2. The nodes whose ranges cross a chunk boundary (a few per boundary):
+//      a thread starts at each such node without crossing children and
+//      climbs with device-scope arrival counters.
+constexpr int kRefitChunk = 1024;
+
+__device__ __forceinline__ bool refit_crossing(uint2 r) { return r.x / kRefitChunk != r.y / kRefitChunk; }
+
+// Builds node p's traversal record and AABB (max weight in lo.w) from its
+// children; box_of(ch) gives an internal child's AABB.
+template <typename BoxOf>
+__device__ __forceinline__ void refit_node(uint32_t p, const float4 *__restrict__ pts, const uint32_t *__restrict__ children,
+                                           const uint2 *__restrict__ ranges, int leaf_size, Node *__restrict__ nodes,
+                                           BoxOf box_of, float4 &blo, float4 &bhi) {
+    float4 lo[2], hi[2];
+    uint32_t ref[2];
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t ch = children[2 * p + c];
+        uint32_t first, last;
+        if (ch & 0x80000000u) {
+            const uint32_t prim = ch & 0x7fffffffu;
+            const float4 q = pts[prim];
+            lo[c] = q;
+            hi[c] = make_float4(q.x, q.y, q.z, 0.0f);
+            first = last = prim;
+        } else {
+            box_of(ch, lo[c], hi[c]);
+            const uint2 r = ranges[ch];
+            first = r.x;
+            last = r.y;
         }
-        Node nd;
-        nd.lo0 = lo[0];
-        nd.hi0 = make_float4(hi[0].x, hi[0].y, hi[0].z, __uint_as_float(ref[0]));
-        nd.lo1 = lo[1];
-        nd.hi1 = make_float4(hi[1].x, hi[1].y, hi[1].z, __uint_as_float(ref[1]));
-        nodes[p] = nd;
-        boxes[2 * (int64_t)p] = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z),
-                                            fmaxf(lo[0].w, lo[1].w));
-        boxes[2 * (int64_t)p + 1] =
-            make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
+        const uint32_t count = last - first + 1;
+        ref[c] = (count <= (uint32_t)leaf_size) ? leaf_ref(first, count) : ch;
+    }
+    Node nd;
+    nd.lo0 = lo[0];
+    nd.hi0 = make_float4(hi[0].x, hi[0].y, hi[0].z, __uint_as_float(ref[0]));
+    nd.lo1 = lo[1];
+    nd.hi1 = make_float4(hi[1].x, hi[1].y, hi[1].z, __uint_as_float(ref[1]));
+    nodes[p] = nd;
+    blo = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z), fmaxf(lo[0].w, lo[1].w));
+    bhi = make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
+}
+
+__global__ void __launch_bounds__(kRefitChunk)
+    refit_chunk_kernel(const float4 *__restrict__ pts, int64_t n, const uint32_t *__restrict__ children,
+                       const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges, float4 *__restrict__ boxes,
+                       Node *__restrict__ nodes, int leaf_size) {
+    __shared__ float4 slo[kRefitChunk], shi[kRefitChunk];
+    __shared__ uint32_t arrive[kRefitChunk];
+    const int64_t c0 = (int64_t)blockIdx.x * kRefitChunk;
+    const int64_t c1 = (c0 + kRefitChunk < n ? c0 + kRefitChunk : n) - 1;  // chunk points [c0, c1]
+    arrive[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t t = c0 + threadIdx.x;
+    if (t > c1) return;
+    uint32_t p = parents[(n - 1) + t];
+    auto box_of = [&](uint32_t ch, float4 &lo, float4 &hi) {
+        lo = slo[ch - c0];
+        hi = shi[ch - c0];
+    };
+    while (p != 0xFFFFFFFFu && (int64_t)p < n - 1) {
+        const uint2 r = ranges[p];
+        if ((int64_t)r.x < c0 || (int64_t)r.y > c1) return;  // crosses the chunk: phase 2
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&arrive[p - c0], 1u) == 0u) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        float4 blo, bhi;
+        refit_node(p, pts, children, ranges, leaf_size, nodes, box_of, blo, bhi);
+        slo[p - c0] = blo;
+        shi[p - c0] = bhi;
+        boxes[2 * (int64_t)p] = blo;
+        boxes[2 * (int64_t)p + 1] = bhi;
         p = parents[p];
+    }
+}
+
+__device__ __forceinline__ int refit_crossing_children(uint32_t p, const uint32_t *__restrict__ children,
+                                                       const uint2 *__restrict__ ranges) {
+    int k = 0;
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t ch = children[2 * p + c];
+        if (!(ch & 0x80000000u) && refit_crossing(ranges[ch])) k++;
+    }
+    return k;
+}
+
+__global__ void refit_top_kernel(const float4 *__restrict__ pts, int64_t n, const uint32_t *__restrict__ children,
+                                 const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
+                                 float4 *__restrict__ boxes, uint32_t *__restrict__ flags, Node *__restrict__ nodes,
+                                 int leaf_size) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (!refit_crossing(ranges[i]) || refit_crossing_children((uint32_t)i, children, ranges) != 0) return;
+    auto box_of = [&](uint32_t ch, float4 &lo, float4 &hi) {
+        const float *b = (const float *)&boxes[2 * (int64_t)ch];
+        lo = make_float4(load_coherent(b + 0), load_coherent(b + 1), load_coherent(b + 2), load_coherent(b + 3));
+        hi = make_float4(load_coherent(b + 4), load_coherent(b + 5), load_coherent(b + 6), 0.0f);
+    };
+    uint32_t p = (uint32_t)i;
+    // a crossing path is at most as deep as the hierarchy (< 64 levels)
+    for (int level = 0; level < 128; ++level) {
+        float4 blo, bhi;
+        refit_node(p, pts, children, ranges, leaf_size, nodes, box_of, blo, bhi);
+        boxes[2 * (int64_t)p] = blo;
+        boxes[2 * (int64_t)p + 1] = bhi;
+        const uint32_t q = parents[p];
+        if (q == 0xFFFFFFFFu || (int64_t)q >= n - 1) return;
+        if (refit_crossing_children(q, children, ranges) == 2) {  // the second arrival builds q
+            __threadfence();
+            if (atomicAdd(&flags[q], 1u) == 0u) return;
+            __threadfence();
+        }
+        p = q;
     }
 }
 
@@ -411,8 +483,11 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
         hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, keys, n, children, parents,
                            ranges);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, flags, n - 1, 0u);
-        hipLaunchKernelGGL(refit_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted, n, children, parents,
-                           ranges, boxes, flags, nodes, leaf_size);
+        hipLaunchKernelGGL(refit_chunk_kernel, dim3(grid_for(n, kRefitChunk)), dim3(kRefitChunk), 0, stream, sorted, n,
+                           children, parents, ranges, boxes, nodes, leaf_size);
+        if (n > kRefitChunk)
+            hipLaunchKernelGGL(refit_top_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, sorted, n, children,
+                               parents, ranges, boxes, flags, nodes, leaf_size);
     }
     return hipGetLastError();
 }
