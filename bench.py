@@ -30,15 +30,67 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mcells/s @1M & 10M pts, plain vs learned-radius form; % cells exact vs FP64"
 
 
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
+CELL_KERNELS = ("cell_group_kernel", "cell_group_exact_kernel", "cell_group_big_kernel", "cell_group_huge_kernel")
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
 def _traffic_record(workload_key: str):
-    """HBM bytes per cell-kernel launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
+    """HBM bytes per cell-construction launch from the committed rocprofv3 PMC summary, if any."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_PATH) as f:
             rec = json.load(f)
         return rec.get(workload_key)
     except (OSError, ValueError):
         return None
+
+
+def _pmc_values(run_dir: str, counter: str):
+    """{dispatch id: (kernel name, value)} of one counter from a rocprofv3 --pmc csv run."""
+    import csv
+    import glob
+
+    out = {}
+    for path in glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    out[int(r["Dispatch_Id"])] = (r["Kernel_Name"], float(r["Counter_Value"]))
+    return out
+
+
+def summarize_pmc(fetch_dir: str, write_dir: str, copy_mib: int, workload_key: str, command: str) -> dict:
+    """Cell-construction HBM bytes per launch from two --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch), each scaled by the same counter's reading for a device copy of copy_mib MiB."""
+    res = {}
+    for counter, run_dir in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        vals = _pmc_values(run_dir, counter)
+        copies = [v for name, v in vals.values() if name.startswith(COPY_KERNEL)]
+        if not copies:
+            raise RuntimeError(f"no {COPY_KERNEL} dispatch in {run_dir}")
+        factor = max(copies) / (copy_mib * 1024.0)  # counter KiB per byte-KiB moved by the copy
+        per_kernel = {}
+        for name, v in vals.values():
+            short = name.split("(")[0].split("::")[-1]
+            if short in CELL_KERNELS:
+                per_kernel.setdefault(short, []).append(v)
+        launches = len(per_kernel.get(CELL_KERNELS[0], []))
+        kib = sum(sum(v) for v in per_kernel.values()) / max(launches, 1)
+        res[counter] = {"kib_per_launch_raw": round(kib, 1), "copy_factor": round(factor, 4),
+                        "bytes_per_launch": round(kib * 1024.0 / factor)}
+    total = res["FETCH_SIZE"]["bytes_per_launch"] + res["WRITE_SIZE"]["bytes_per_launch"]
+    rec = {"bytes_per_launch": total, "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"],
+           "kernels": list(CELL_KERNELS), "copy_mib": copy_mib, "command": command}
+    try:
+        with open(TRAFFIC_PATH) as f:
+            allrec = json.load(f)
+    except (OSError, ValueError):
+        allrec = {}
+    allrec[workload_key] = rec
+    with open(TRAFFIC_PATH, "w") as f:
+        json.dump(allrec, f, indent=1)
+        f.write("\n")
+    return rec
 
 
 def main() -> int:
@@ -50,7 +102,23 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="cells timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed steps, one device copy of this many MiB (PMC calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this command "
+                         "with --copy-mib into profiles/cell_kernel_traffic.json and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        if args.copy_mib <= 0:
+            print("bench.py: --pmc-summary needs the --copy-mib of the profiled runs", file=sys.stderr)
+            return 2
+        cmd = ("rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d <dir> -o run -- "
+               f"python bench.py --n {args.n} --seed {args.seed} --steps {args.steps} --warmup {args.warmup} "
+               f"--no-cpu --copy-mib {args.copy_mib}")
+        rec = summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_mib,
+                            f"uniform_{args.n}_seed{args.seed}", cmd)
+        print(json.dumps(rec))
+        return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is "
               "described in DESIGN.md and not built)" % args.gpus, file=sys.stderr)
@@ -84,6 +152,13 @@ def main() -> int:
     ms_per_step = elapsed * 1e3 / args.steps
     value = args.n * args.steps / elapsed / 1e6
 
+    if args.copy_mib > 0:  # a plain copy of known size for the PMC byte counters
+        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     adj, off, status = (t.cpu().numpy() for t in out)
     e_total = int(off[-1])
     f_bar = e_total / args.n
@@ -92,7 +167,8 @@ def main() -> int:
     kern_s = float(np.mean(kern_ms)) / 1e3
     achieved = args.n * b_cell / kern_s / 1e9
     wkey = f"uniform_{args.n}_seed{args.seed}"
-    traffic = _traffic_record(wkey)
+    trec = _traffic_record(wkey)
+    traffic = trec["bytes_per_launch"] if trec else None
 
     cpu = None
     exact = None
@@ -137,7 +213,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic,
-                     "kernel": "pg::cell_kernel", "kernel_ms": round(kern_s * 1e3, 4),
+                     "kernel": "cell construction: pg::" + " + pg::".join(CELL_KERNELS),
+                     "kernel_ms": round(kern_s * 1e3, 4),
                      "bytes_per_cell": round(b_cell, 2)},
         "cpu_baseline": cpu,
         "exact_cells": exact,

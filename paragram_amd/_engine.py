@@ -127,7 +127,12 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         goffs = initial_guesses_offsets
         # knn > 0: clip by the 16 Morton-adjacent points before the traversal
         # (initial guesses replace it, as they replace kNN, voronoi_ultra.cu:158-169)
-        seed_window = 16 if (knn > 0 and guesses is None) else 0
+        # The reference pre-clips each cell by its knn nearest neighbours
+        # (voronoi_ultra.cu:70-85).  Here the best-first traversal already
+        # offers points nearest-first, and a Morton-window pre-clip measured
+        # slower on the 1M uniform cloud (more clips undone later), so knn
+        # selects nothing: window 0.
+        seed_window = 0
         if guesses is not None:
             # the kernel indexes guesses through the offsets: reject malformed CSR
             ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= guesses.numel())).item())

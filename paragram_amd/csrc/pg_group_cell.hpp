@@ -842,9 +842,15 @@ struct Group {
             C->aH[gl] = make_float4(c.ax, c.ay, c.az, c.H);
             C->po[gl] = make_float2(c.pH, __int_as_float(c.owner));
         }
+        float hkey = live_c ? c.H : FLT_MAX;
         while (todo) {
-            const int b = (int)__builtin_ctzll(todo);
-            todo &= todo - 1ull;
+            // nearest plane first (smallest H = power distance of the point):
+            // the cell shrinks fastest and fewer clips are undone later
+            const float hmin = gmin(hkey);
+            const uint64_t at = gballot(live_c && hkey == hmin) & todo;
+            const int b = (int)__builtin_ctzll(at ? at : todo);
+            todo &= ~(1ull << b);
+            if (gl == b) hkey = FLT_MAX;
             Cand q;
             const float4 aH = C->aH[b];
             const float2 po = C->po[b];
