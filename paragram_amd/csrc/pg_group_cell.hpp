@@ -97,19 +97,16 @@ __device__ __forceinline__ int mask_select(const uint64_t (&m)[W], int j) {
     }
     return base;
 }
-// the lowest n set bits of m
+// the lowest n set bits of m (n <= popcount)
 template <int W>
 __device__ __forceinline__ void mask_first(const uint64_t (&m)[W], int n, uint64_t (&out)[W]) {
+    const int total = mask_popc(m);
+    const int cut = n < total ? mask_select(m, n) : 64 * W;  // first bit not taken
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        uint64_t x = m[w], keep = 0;
-        while (n > 0 && x) {
-            const uint64_t low = x & (~x + 1ull);
-            keep |= low;
-            x ^= low;
-            n--;
-        }
-        out[w] = keep;
+        const int lo = 64 * w;
+        const uint64_t below = cut >= lo + 64 ? ~0ull : (cut <= lo ? 0ull : ((1ull << (cut - lo)) - 1ull));
+        out[w] = m[w] & below;
     }
 }
 
@@ -838,6 +835,7 @@ struct Group {
         const uint64_t skipped = gballot(has && !live_c);
         if (gl == 0 && skipped) count(kStatSkipped, (uint32_t)__popcll(skipped));
         uint64_t todo = gballot(live_c);
+        if (gl == 0 && todo) count(kStatCandidates, (uint32_t)__popcll(todo));
         if (live_c) {
             C->aH[gl] = make_float4(c.ax, c.ay, c.az, c.H);
             C->po[gl] = make_float2(c.pH, __int_as_float(c.owner));
@@ -862,7 +860,6 @@ struct Group {
             q.pa1 = fabsf(q.ax) + fabsf(q.ay) + fabsf(q.az);
             q.owner = __float_as_int(po.y);
             if (bloom && has_plane(q.owner)) continue;
-            if (gl == 0) count(kStatCandidates);
             if (clip(q)) {
                 if (gl == 0) count(kStatClips);
                 if (status != kSuccess) return;
