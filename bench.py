@@ -30,6 +30,15 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mcells/s @1M & 10M pts, plain vs learned-radius form; % cells exact vs FP64"
 
 
+# name -> (generator in paragram_amd.workloads, default n, default seed, description)
+WORKLOADS = {
+    "c1": ("uniform", 10_000, 0, "plain Voronoi, {n:,} points fp32 i.i.d. U[0,1)^3, seed {seed}"),
+    "c2": ("uniform", 1_000_000, 1, "plain Voronoi, {n:,} points fp32 i.i.d. U[0,1)^3, seed {seed}"),
+    "c3": ("clustered_power", 2_000_000, 2, "power diagram, {n:,} points fp32: 90% from 64 Gaussians "
+           "(sigma 0.02, clamped) + 10% uniform, weights U[0, 3e-5], seed {seed}"),
+    "c5": ("learned_power", 10_000_000, 4, "power diagram, {n:,} points fp32: 50% uniform + 50% from 256 "
+           "Gaussians (sigma 0.01), weights U[0, 5e-6], seed {seed} (plain form)"),
+}
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
 CELL_KERNELS = ("cell_group_kernel", "cell_group_exact_kernel", "cell_group_big_kernel", "cell_group_huge_kernel")
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
@@ -98,8 +107,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
+                    help="c2 (default, the headline configuration), or another configuration of the metric")
+    ap.add_argument("--n", type=int, default=None, help="override the workload's point count")
+    ap.add_argument("--seed", type=int, default=None, help="override the workload's seed")
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="cells timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -108,6 +119,10 @@ def main() -> int:
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this command "
                          "with --copy-mib into profiles/cell_kernel_traffic.json and exit")
     args = ap.parse_args()
+    gen, n_def, seed_def, desc = WORKLOADS[args.workload]
+    args.n = n_def if args.n is None else args.n
+    args.seed = seed_def if args.seed is None else args.seed
+    wkey = f"{gen}_{args.n}_seed{args.seed}"
     if args.pmc_summary:
         if args.copy_mib <= 0:
             print("bench.py: --pmc-summary needs the --copy-mib of the profiled runs", file=sys.stderr)
@@ -115,8 +130,7 @@ def main() -> int:
         cmd = ("rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d <dir> -o run -- "
                f"python bench.py --n {args.n} --seed {args.seed} --steps {args.steps} --warmup {args.warmup} "
                f"--no-cpu --copy-mib {args.copy_mib}")
-        rec = summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_mib,
-                            f"uniform_{args.n}_seed{args.seed}", cmd)
+        rec = summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_mib, wkey, cmd)
         print(json.dumps(rec))
         return 0
     if args.gpus != 1:
@@ -133,12 +147,14 @@ def main() -> int:
     if _backend._C is None:
         raise RuntimeError("HIP library not built: " + str(_backend._load_error))
     dev = torch.device("cuda:0")
-    pts_np = workloads.uniform(args.n, args.seed)
+    made = getattr(workloads, gen)(args.n, args.seed)
+    pts_np, w_np = made if isinstance(made, tuple) else (made, None)
     pts = torch.from_numpy(pts_np).to(dev)
+    w = None if w_np is None else torch.from_numpy(w_np).to(dev).reshape(-1, 1)
     lib = _backend._C.lib
 
     def step(instr=None):
-        return _engine.build_diagram(lib, pts, None, 8, False, None, None, -1, instrument=instr)
+        return _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr)
 
     for _ in range(args.warmup):
         step()
@@ -166,7 +182,6 @@ def main() -> int:
     kern_ms = [a.elapsed_time(b) for a, b in instr.cell_events]
     kern_s = float(np.mean(kern_ms)) / 1e3
     achieved = args.n * b_cell / kern_s / 1e9
-    wkey = f"uniform_{args.n}_seed{args.seed}"
     trec = _traffic_record(wkey)
     traffic = trec["bytes_per_launch"] if trec else None
 
@@ -179,10 +194,10 @@ def main() -> int:
         sample = workloads.oracle_sample(args.n, args.seed, args.cpu_sample)
         # the oracle builds its k-d tree per call: time that separately
         tb = time.perf_counter()
-        oracle.cells(pts_np, None, sample[:1], nthreads=cores)
+        oracle.cells(pts_np, w_np, sample[:1], nthreads=cores)
         t_build = time.perf_counter() - tb
         tc = time.perf_counter()
-        ref = oracle.cells(pts_np, None, sample, nthreads=cores)
+        ref = oracle.cells(pts_np, w_np, sample, nthreads=cores)
         t_cells = time.perf_counter() - tc - t_build
         cpu = {"value": round(sample.shape[0] / max(t_cells, 1e-9) / 1e6, 4), "unit": "Mcells/s",
                "cores": cores, "kind": "port",
@@ -208,8 +223,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain Voronoi, {args.n:,} points fp32 i.i.d. U[0,1)^3, seed {args.seed}",
-                   "form": "plain", "n_points": args.n, "edges": e_total, "mean_faces": round(f_bar, 4)},
+        "config": {"workload": desc.format(n=args.n, seed=args.seed), "form": "plain", "n_points": args.n,
+                   "edges": e_total, "mean_faces": round(f_bar, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic,

@@ -147,3 +147,24 @@ def test_c2_voronoi_1m_full_oracle(gpu):
     pts = workloads.uniform(1_000_000, 1)
     got = run(gpu, pts)
     assert_matches_oracle(got, pts)
+
+
+def test_c3_power_2m_full_oracle(gpu):
+    log("C3 power 2M (clustered, hidden cells) full oracle")
+    pts, w = workloads.clustered_power(2_000_000, 2)
+    got = run(gpu, pts, w)
+    csr_invariants(*got)
+    assert (got[2] == 3).any(), "C3 is built to contain hidden cells"
+    assert_matches_oracle(got, pts, w)
+
+
+def test_c5_learned_power_10m_sample(gpu):
+    log("C5 power 10M, 100k-cell seeded oracle sample")
+    pts, w = workloads.learned_power(10_000_000, 4)
+    adj, off, st = run(gpu, pts, w)
+    sample = workloads.oracle_sample(pts.shape[0], 4, 100_000)
+    ref = oracle.cells(pts, w, sample)
+    bad = 0
+    for k, i in enumerate(sample):
+        bad += int(st[i] != ref.status[k] or not np.array_equal(adj[off[i]:off[i + 1]], ref.row(k)))
+    assert bad == 0, f"{bad} of {sample.shape[0]} sampled cells differ"
