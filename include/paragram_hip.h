@@ -100,6 +100,40 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream);
 
+/* ---- Batches of clouds --------------------------------------------------
+ * No counterpart in the reference, which builds one cloud per call
+ * (voronoi_ultra.cu:143-376, laguerre_ultra.cu:176-454); for callers with many
+ * clouds (e.g. a stream of LiDAR scans) that want one launch for all of them.
+ * The clouds are contiguous ranges of one points array: cloud c is
+ * cloud_offsets[c] .. cloud_offsets[c + 1] (host array of n_clouds + 1
+ * int64, starting at 0 and ending at n, non-decreasing; 1 <= n_clouds <=
+ * 4096).  Each cloud's cells, box (its own AABB +- 1) and neighbour ids are
+ * those of a single-cloud call on that range; ids are global indices into
+ * the points array (subtract the cloud's first index for cloud-local ids).
+ * The CSR calls below (scan, symmetry check, emission) work unchanged on
+ * the batch as one diagram over all n points. */
+
+/* Bytes of the batch index workspace. */
+int pg_batch_workspace_bytes(int64_t n, int32_t n_clouds, uint64_t *bytes);
+
+/* Builds the batch index (per-cloud AABBs and boxes, one sort and hierarchy
+ * over keys cloud id || per-cloud Morton code, each cloud's subtree root). */
+int pg_build_batch_index(int64_t n, int32_t n_clouds, const float *points, const float *weights,
+                         const int64_t *cloud_offsets, int32_t leaf_size, void *workspace, uint64_t workspace_bytes,
+                         void *stream);
+
+/* Host helper: first work quad (four consecutive cells of one cloud) of each
+ * cloud within the sorted cell range [begin, end); quads[n_clouds] = total. */
+int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin, int64_t end, int64_t *quads);
+
+/* pg_compute_cells over the cells [begin, end) of a batch (any range of the
+ * batch, so the chunk plan applies unchanged); the persistent waves take one
+ * quad per round from a global counter. */
+int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
+                           int64_t begin, int64_t end, int32_t *rows, int64_t *row_offsets, int32_t *counts,
+                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
+                           uint32_t *cell_cycles, void *stream);
+
 /* Bytes of temporary storage for pg_scan_counts over n values. */
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes);
 

@@ -139,3 +139,55 @@ def voronoi_diagram(
         leaf_size=leaf_size,
     )
     return Diagram(adjacency=adjacency, offsets=offsets, status=status)
+
+
+# ---------------------------------------------------------------- batches
+# No counterpart in the reference (one cloud per call there): many clouds in
+# one persistent launch, each cloud's result equal to its single call.
+
+def _validate_cloud_offsets(cloud_offsets, num_points: int) -> "np.ndarray":
+    import numpy as np
+
+    if isinstance(cloud_offsets, torch.Tensor):
+        cloud_offsets = cloud_offsets.detach().cpu().numpy()
+    off = np.asarray(cloud_offsets)
+    if off.ndim != 1 or off.shape[0] < 2 or not np.issubdtype(off.dtype, np.integer):
+        raise ValueError("cloud_offsets must be a 1-D integer sequence of length (B + 1,) with B >= 1")
+    off = off.astype(np.int64)
+    if off.shape[0] - 1 > 4096:
+        raise ValueError("at most 4096 clouds per batch")
+    if off[0] != 0 or off[-1] != num_points or (np.diff(off) < 0).any():
+        raise ValueError("cloud_offsets must start at 0, end at N and be non-decreasing")
+    return off
+
+
+def _batch(points, weights, cloud_offsets, debug, leaf_size):
+    from ._backend import _C, _load_error
+    from ._engine import build_diagram, split_batch
+
+    if _C is None:
+        raise ImportError(f"paragram_amd HIP engine unavailable: {_load_error}")
+    off = _validate_cloud_offsets(cloud_offsets, points.shape[0])
+    adjacency, offsets, status = build_diagram(_C.lib, points, weights, 0, debug, None, None, leaf_size, clouds=off)
+    return [Diagram(adjacency=a, offsets=o, status=s) for a, o, s in split_batch(adjacency, offsets, status, off)]
+
+
+def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets, *, debug: bool = False,
+                            leaf_size: int = -1) -> list:
+    """Voronoi diagrams of a batch of clouds in one launch.
+
+    ``points`` (N, 3) float32 holds the clouds back to back; cloud c is rows
+    ``cloud_offsets[c]`` .. ``cloud_offsets[c + 1]``.  Returns one ``Diagram``
+    per cloud, equal to ``voronoi_diagram`` of that cloud alone (cloud-local
+    ids, each cloud clipped to its own padded box).
+    """
+    points = _validate_points(points)
+    return _batch(points, None, cloud_offsets, debug, leaf_size)
+
+
+def batched_power_diagram(points: torch.Tensor, weights: torch.Tensor, cloud_offsets, *, debug: bool = False,
+                          leaf_size: int = -1) -> list:
+    """Power diagrams of a batch of weighted clouds (see batched_voronoi_diagram)."""
+    points = _validate_points(points)
+    weights = _validate_weights(weights, points.shape[0], points.device)
+    return _batch(points, weights, cloud_offsets, debug, leaf_size)

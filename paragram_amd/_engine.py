@@ -14,6 +14,7 @@ import ctypes
 import os
 import time
 
+import numpy as np
 import torch
 
 from ._chunking import plan_chunks
@@ -57,6 +58,11 @@ class HipLibrary:
             "pg_emit_csr": ([i64, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
             "pg_symmetrize_union": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], ctypes.c_int),
             "pg_fill_u32": ([vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
+            "pg_batch_workspace_bytes": ([i64, i32, u64p], ctypes.c_int),
+            "pg_build_batch_index": ([i64, i32, vp, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
+            "pg_batch_quads": ([i32, vp, i64, i64, vp], ctypes.c_int),
+            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp,
+                                        vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -93,9 +99,20 @@ class Instrument:
 
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
                   initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
-                  instrument: Instrument | None = None):
-    """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device."""
+                  instrument: Instrument | None = None, clouds=None):
+    """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device.
+
+    ``clouds``: optional host int64 array of cloud offsets (a batch of clouds,
+    contiguous ranges of ``points``); the result is then the batch as one
+    diagram over all points, neighbour ids global.
+    """
     n = int(points.shape[0])
+    if clouds is not None:
+        clouds = np.ascontiguousarray(clouds, dtype=np.int64)
+        n_clouds = int(clouds.shape[0]) - 1
+        c_off = ctypes.c_void_p(clouds.ctypes.data)
+        if initial_guesses is not None:
+            raise ValueError("initial guesses are not supported for batches of clouds")
     dev = points.device
     i32 = dict(dtype=torch.int32, device=dev)
     if n == 0:  # the reference launches a zero-size grid here; return the empty diagram
@@ -106,9 +123,15 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         if debug:
             torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        ws_bytes = lib.u64("pg_index_workspace_bytes", n)
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-        lib.call("pg_build_index", n, _p(points), _p(weights), int(leaf_size), _p(ws), ws_bytes, sp)
+        if clouds is None:
+            ws_bytes = lib.u64("pg_index_workspace_bytes", n)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+            lib.call("pg_build_index", n, _p(points), _p(weights), int(leaf_size), _p(ws), ws_bytes, sp)
+        else:
+            ws_bytes = lib.u64("pg_batch_workspace_bytes", n, n_clouds)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+            lib.call("pg_build_batch_index", n, n_clouds, _p(points), _p(weights), c_off, int(leaf_size), _p(ws),
+                     ws_bytes, sp)
         if debug:
             torch.cuda.synchronize(dev)
             print("BVH Creation time: %f ms" % ((time.perf_counter() - t0) * 1e3))
@@ -125,8 +148,6 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
 
         guesses = initial_guesses
         goffs = initial_guesses_offsets
-        # knn > 0: clip by the 16 Morton-adjacent points before the traversal
-        # (initial guesses replace it, as they replace kNN, voronoi_ultra.cu:158-169)
         # The reference pre-clips each cell by its knn nearest neighbours
         # (voronoi_ultra.cu:70-85).  Here the best-first traversal already
         # offers points nearest-first, and a Morton-window pre-clip measured
@@ -156,9 +177,14 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             if instrument is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(stream)
-            lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, seed_window, _p(rows),
-                     _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves,
-                     _p(instrument.cycles if instrument is not None else None), sp)
+            cyc = _p(instrument.cycles if instrument is not None else None)
+            if clouds is None:
+                lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, seed_window,
+                         _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes,
+                         waves, cyc, sp)
+            else:
+                lib.call("pg_compute_cells_batch", n, _p(ws), n_clouds, c_off, begin, end, _p(rows), _p(row_offsets),
+                         _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
             if ev is not None:
                 ev[1].record(stream)
                 instrument.cell_events.append(ev)
@@ -210,6 +236,43 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
             instrument.plan = plan
         return adjacency, offsets, status
+
+
+def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds):
+    """Per-cloud ``(adjacency, offsets, status)`` of a batch diagram, with
+    cloud-local ids; views into two new device arrays, no host round trip."""
+    dev = offsets.device
+    clouds = np.ascontiguousarray(clouds, dtype=np.int64)
+    b = clouds.shape[0] - 1
+    starts = torch.from_numpy(clouds[:-1].copy()).to(dev)
+    ends = torch.from_numpy(clouds[1:].copy()).to(dev)
+    off64 = offsets.to(torch.int64)
+    seg = off64[ends] - off64[starts]  # entries per cloud
+    e_total = int(adjacency.numel())
+    local_adj = (adjacency.to(torch.int64) - torch.repeat_interleave(starts, seg, output_size=e_total)).to(torch.int32)
+    # cloud c's offsets are offsets[a_c .. b_c] - offsets[a_c], laid out one
+    # cloud after another (n_c + 1 entries each)
+    lens = clouds[1:] - clouds[:-1] + 1
+    firsts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    pos = torch.arange(int(lens.sum()), device=dev)
+    cid = torch.searchsorted(torch.from_numpy(firsts).to(dev), pos, right=True) - 1
+    row = pos - torch.from_numpy(firsts).to(dev)[cid] + starts[cid]
+    local_off = (off64[row] - off64[starts[cid]]).to(torch.int32)
+    out = []
+    e0 = 0
+    seg_host = None
+    for c in range(b):
+        a, z = int(clouds[c]), int(clouds[c + 1])
+        o = local_off[int(firsts[c]):int(firsts[c]) + (z - a) + 1]
+        out.append((o, status[a:z]))
+    # adjacency slices need the per-cloud entry counts on the host (one read)
+    seg_host = seg.cpu().numpy()
+    res = []
+    for c in range(b):
+        m = int(seg_host[c])
+        res.append((local_adj[e0:e0 + m], out[c][0], out[c][1]))
+        e0 += m
+    return res
 
 
 def _union(lib, n, counts, offsets, adjacency, scan_tmp, sp, dev):

@@ -7,6 +7,7 @@
 
 #include <stdio.h>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -57,6 +58,60 @@ int pg_build_index(int64_t n, const float *points, const float *weights, int32_t
     if (leaf < 1) leaf = 1;
     if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
     return check_hip(pg::build_index(n, points, weights, leaf, workspace, as_stream(stream)), "build_index");
+}
+
+int pg_batch_workspace_bytes(int64_t n, int32_t n_clouds, uint64_t *bytes) {
+    if (int r = check_n(n)) return r;
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (!bytes) return fail(1, "bytes is null");
+    *bytes = pg::make_batch_layout(n, n_clouds).total;
+    return 0;
+}
+
+namespace {
+int check_cloud_offsets(int64_t n, int32_t n_clouds, const int64_t *off) {
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (!off) return fail(1, "cloud_offsets is null");
+    if (off[0] != 0 || off[n_clouds] != n) return fail(1, "cloud_offsets must start at 0 and end at n");
+    for (int32_t c = 0; c < n_clouds; ++c)
+        if (off[c + 1] < off[c]) return fail(1, "cloud_offsets must be non-decreasing");
+    return 0;
+}
+}  // namespace
+
+int pg_build_batch_index(int64_t n, int32_t n_clouds, const float *points, const float *weights,
+                         const int64_t *cloud_offsets, int32_t leaf_size, void *workspace, uint64_t workspace_bytes,
+                         void *stream) {
+    if (int r = check_n(n)) return r;
+    if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
+    if (n == 0) return 0;
+    if (!points || !workspace) return fail(1, "points and workspace must be non-null");
+    const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
+    if (workspace_bytes < B.total) return fail(1, "batch workspace too small");
+    int leaf = leaf_size < 0 ? pg::kDefaultLeaf : leaf_size;
+    if (leaf < 1) leaf = 1;
+    if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
+    int64_t *dev_off = (int64_t *)((char *)workspace + B.offsets);
+    hipError_t e = hipMemcpyAsync(dev_off, cloud_offsets, 8 * ((size_t)n_clouds + 1), hipMemcpyHostToDevice,
+                                  as_stream(stream));
+    if (e != hipSuccess) return check_hip(e, "build_batch_index");
+    return check_hip(pg::build_batch_index(n, n_clouds, points, weights, dev_off, leaf, workspace, as_stream(stream)),
+                     "build_batch_index");
+}
+
+int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin, int64_t end, int64_t *quads) {
+    if (!cloud_offsets || !quads) return fail(1, "cloud_offsets and quads must be non-null");
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (begin < 0 || end < begin || end > cloud_offsets[n_clouds]) return fail(1, "cell range out of bounds");
+    int64_t q = 0;
+    for (int32_t c = 0; c < n_clouds; ++c) {
+        quads[c] = q;
+        const int64_t lo = cloud_offsets[c] > begin ? cloud_offsets[c] : begin;
+        const int64_t hi = cloud_offsets[c + 1] < end ? cloud_offsets[c + 1] : end;
+        if (hi > lo) q += (hi - lo + 3) / 4;
+    }
+    quads[n_clouds] = q;
+    return 0;
 }
 
 int pg_default_waves(int32_t *waves) {
@@ -127,6 +182,60 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
     a.huge_cap = huge_cap;
     a.cycles = cell_cycles;
     return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells");
+}
+
+int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
+                           int64_t begin, int64_t end, int32_t *rows, int64_t *row_offsets, int32_t *counts,
+                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
+                           uint32_t *cell_cycles, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
+    if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
+    if (begin == end) return 0;
+    if (!workspace || !rows || !row_offsets || !counts || !status || !stats || !scratch)
+        return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
+    if (waves < 1) return fail(1, "waves must be positive");
+    const int w0 = (waves + 3) / 4 * 4;
+    const int64_t big_cap = pg::tier1_capacity(end - begin);
+    const int64_t huge_cap = pg::tier2_capacity(end - begin);
+    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap, end - begin))
+        return fail(1, "cell scratch too small");
+    const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
+    char *ws = (char *)workspace;
+    std::vector<int64_t> quads((size_t)n_clouds + 1);
+    if (int r = pg_batch_quads(n_clouds, cloud_offsets, begin, end, quads.data())) return r;
+    // the copy is ordered before the kernels on the stream; hipMemcpyAsync
+    // returns once the pageable source has been staged
+    hipError_t e = hipMemcpyAsync(ws + B.quads, quads.data(), 8 * quads.size(), hipMemcpyHostToDevice,
+                                  as_stream(stream));
+    if (e != hipSuccess) return check_hip(e, "compute_cells_batch");
+    pg::CellArgs a;
+    a.n = n;
+    a.index_ws = workspace;
+    a.begin = begin;
+    a.end = end;
+    a.guesses = nullptr;
+    a.guess_offsets = nullptr;
+    a.guesses_sorted = 0;
+    a.seed_window = 0;
+    a.rows = rows;
+    a.row_stride = pg::kRowCap;
+    a.row_offsets = row_offsets;
+    a.counts = counts;
+    a.status = status;
+    a.stats = (unsigned long long *)stats;
+    a.scratch = scratch;
+    a.scratch_bytes = scratch_bytes;
+    a.waves = w0;
+    a.waves1 = pg::kTier1Waves;
+    a.big_cap = big_cap;
+    a.huge_cap = huge_cap;
+    a.cycles = cell_cycles;
+    a.clouds = (const pg::IndexHeader *)(ws + B.clouds);
+    a.cloud_cells = (const int64_t *)(ws + B.offsets);
+    a.quads = (const int64_t *)(ws + B.quads);
+    a.n_clouds = n_clouds;
+    return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells_batch");
 }
 
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes) {

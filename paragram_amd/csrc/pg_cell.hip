@@ -66,6 +66,11 @@ struct CellParams {
     uint32_t huge_cap;
     uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
     uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
+    // batch of clouds (null clouds: one cloud described by hdr)
+    const IndexHeader *clouds;    // [n_clouds] per-cloud box and subtree root
+    const int64_t *cloud_cells;   // [n_clouds + 1] sorted-cell range of each cloud
+    const int64_t *quads;         // [n_clouds + 1] first work quad of each cloud within [begin, end)
+    int n_clouds;
 };
 
 // A vertex is a triple of plane slots packed kBits bits apiece.
@@ -133,6 +138,10 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.status = a.status;
     prm.stats = a.stats;
     prm.cycles = a.cycles;
+    prm.clouds = a.clouds;
+    prm.cloud_cells = a.cloud_cells;
+    prm.quads = a.quads;
+    prm.n_clouds = a.n_clouds;
     char *sc = (char *)a.scratch;
     prm.counters = (uint32_t *)sc;
     prm.big_list = (uint32_t *)(sc + 256);

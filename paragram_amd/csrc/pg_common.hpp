@@ -163,4 +163,36 @@ inline IndexLayout make_index_layout(int64_t n) {
     return L;
 }
 
+// Batch of clouds in one index: the single-cloud layout over all points,
+// then one IndexHeader per cloud (its AABB, clipping box and subtree root).
+constexpr int kMaxClouds = 1 << 12;
+struct BatchLayout {
+    IndexLayout idx;
+    uint64_t clouds;   // IndexHeader [n_clouds]
+    uint64_t offsets;  // int64 [n_clouds + 1]  cloud ranges (input order = sorted order)
+    uint64_t quads;    // int64 [n_clouds + 1]  work quads of the current cell range
+    uint64_t total;
+};
+inline BatchLayout make_batch_layout(int64_t n, int n_clouds) {
+    BatchLayout B{};
+    const uint64_t nc = (uint64_t)(n_clouds > 0 ? n_clouds : 1);
+    B.idx = make_index_layout(n);
+    B.clouds = B.idx.total;
+    B.offsets = align_up(B.clouds + sizeof(IndexHeader) * nc, 256);
+    B.quads = align_up(B.offsets + 8 * (nc + 1), 256);
+    B.total = align_up(B.quads + 8 * (nc + 1), 256);
+    return B;
+}
+// Sort key of a batch: cloud id in the top cloud_bits bits, then the point's
+// Morton code on a per-cloud cubic grid with bits_per_axis bits per axis.
+inline int batch_cloud_bits(int n_clouds) {
+    int b = 0;
+    while ((1 << b) < n_clouds) ++b;
+    return b;
+}
+inline int batch_axis_bits(int n_clouds) {
+    const int m = (32 - batch_cloud_bits(n_clouds)) / 3;
+    return m < 10 ? m : 10;
+}
+
 }  // namespace pg

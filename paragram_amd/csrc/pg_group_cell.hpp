@@ -1057,7 +1057,7 @@ struct Group {
         if (status != kEmptyCell) {
             uint64_t used[kPW];
             used_planes(used);
-            if (prm.hdr->has_duplicates != 0u) {
+            if (hdr->has_duplicates != 0u) {
                 // every exact duplicate of an owner carries the same face
                 bool full = false;
                 if (gl == 0) {
@@ -1173,9 +1173,14 @@ struct Group {
         perm = prm.perm;
         inv = prm.inv;
         nodes = prm.nodes;
-        root_ref = prm.hdr->root_ref;
-        hdr = prm.hdr;
+        set_cloud(prm.hdr);
         if (gl < 12) S->ctr[gl] = 0u;
+    }
+
+    // The cloud the next cells belong to (its clipping box and subtree root).
+    __device__ __forceinline__ void set_cloud(const IndexHeader *h) {
+        hdr = h;
+        root_ref = h->root_ref;
     }
 
     __device__ __forceinline__ void flush(const CellParams &prm, uint32_t cells) const {
@@ -1190,6 +1195,21 @@ struct Group {
         }
     }
 };
+
+// ------------------------------------------------------------------ batches
+// Largest c < n_clouds with off[c] <= x (off non-decreasing): the cloud of a
+// sorted cell (off = cloud cell offsets) or of a work quad (off = quad
+// offsets; empty clouds are skipped because the search takes the last of
+// equal offsets).
+__device__ __forceinline__ int batch_find(const int64_t *off, int n_clouds, int64_t x) {
+    int lo = 0, hi = n_clouds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // ------------------------------------------------------------------ tiers
 constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Threads = 256;
@@ -1255,12 +1275,29 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         if (g.gl == 0 && !g.bail) cells++;
     };
     if constexpr (!kExactTier) {
-        while (true) {
-            uint32_t c0 = 0;
-            if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kCellsPerWave);
-            c0 = uniform(c0);
-            if ((int64_t)c0 >= count) break;
-            round((int64_t)c0 + grp < count, (int64_t)c0 + grp);
+        if (prm.clouds == nullptr) {
+            while (true) {
+                uint32_t c0 = 0;
+                if (lane == 0) c0 = atomicAdd(&prm.counters[0], (uint32_t)kCellsPerWave);
+                c0 = uniform(c0);
+                if ((int64_t)c0 >= count) break;
+                round((int64_t)c0 + grp < count, (int64_t)c0 + grp);
+            }
+        } else {
+            // a batch: one quad (four consecutive cells of one cloud) per round,
+            // so the cells of a wave share their cloud's subtree
+            const int64_t nq = prm.quads[prm.n_clouds];
+            while (true) {
+                uint32_t q0 = 0;
+                if (lane == 0) q0 = atomicAdd(&prm.counters[0], 1u);
+                const int64_t q = (int64_t)uniform(q0);
+                if (q >= nq) break;
+                const int cl = uniform(batch_find(prm.quads, prm.n_clouds, q));
+                g.set_cloud(&prm.clouds[cl]);
+                const int64_t lo = max(prm.cloud_cells[cl], prm.begin), hi = min(prm.cloud_cells[cl + 1], prm.end);
+                const int64_t cell = lo + kCellsPerWave * (q - prm.quads[cl]) + grp;
+                round(cell < hi, cell - prm.begin);
+            }
         }
     } else {
         while (true) {
@@ -1269,10 +1306,21 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
             const int64_t base = (int64_t)uniform(w) * kWave;
             if (base >= count) break;
             uint64_t bits = __ballot(base + lane < count && prm.exact_list[base + lane] != 0u);
+            // in a batch the cells of a round must share a cloud
+            const int lc = (prm.clouds != nullptr && base + lane < count)
+                               ? batch_find(prm.cloud_cells, prm.n_clouds, prm.begin + base + lane) : 0;
             while (bits) {
-                uint64_t b = bits;  // the grp-th flagged cell of the window
+                uint64_t take = bits;
+                if (prm.clouds != nullptr) {
+                    const int cl = uniform(rl(lc, (int)__builtin_ctzll(bits)));
+                    g.set_cloud(&prm.clouds[cl]);
+                    take = bits & __ballot(lc == cl);
+                }
+                uint64_t b = take;  // the grp-th flagged cell of the round
                 for (int t = 0; t < grp; ++t) b &= b - 1ull;
-                for (int t = 0; t < kCellsPerWave; ++t) bits &= bits - 1ull;
+                uint64_t used = take;
+                for (int t = 0; t < kCellsPerWave; ++t) used &= used - 1ull;
+                bits &= ~(take & ~used);
                 round(b != 0ull, base + (b ? (int64_t)__builtin_ctzll(b) : 0));
             }
         }
@@ -1298,6 +1346,7 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         if (k >= queued) break;
         const uint64_t t0 = clock64();
         const uint32_t cell = prm.big_list[k];
+        if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
         g.start(cell, prm);
         g.traverse();
@@ -1338,6 +1387,7 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
         if (k >= queued) break;
         const uint64_t t0 = clock64();
         const uint32_t cell = prm.huge_list[k];
+        if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
         g.start(cell, prm);
         g.traverse();

@@ -431,10 +431,9 @@ hipError_t fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
-                       hipStream_t stream) {
-    const IndexLayout L = make_index_layout(n);
-    char *base = (char *)ws;
+// Sort, hierarchy and refit over keys already written to the workspace.
+static hipError_t build_tail(int64_t n, const float *points, const float *weights, int leaf_size, char *base,
+                             const IndexLayout &L, hipStream_t stream) {
     IndexHeader *hdr = (IndexHeader *)(base + L.header);
     float4 *sorted = (float4 *)(base + L.points);
     uint32_t *perm = (uint32_t *)(base + L.perm);
@@ -449,15 +448,9 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
     uint2 *ranges = (uint2 *)(base + L.ranges);
     float4 *boxes = (float4 *)(base + L.boxes);
     uint32_t *flags = (uint32_t *)(base + L.flags);
-    float *reduce = (float *)(base + L.reduce);
     uint32_t *scan_tmp = (uint32_t *)(base + L.scan_tmp);
     uint32_t *dup_head = (uint32_t *)(base + L.dup_head);
     uint32_t *dup_next = (uint32_t *)(base + L.dup_next);
-
-    hipLaunchKernelGGL(bbox_partial_kernel, dim3(L.reduce_blocks), dim3(kReduceThreads), 0, stream, points, n, reduce);
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kReduceThreads), 0, stream, reduce, L.reduce_blocks, hdr, n,
-                       weights ? 1 : 0, leaf_size);
-    hipLaunchKernelGGL(morton_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, hdr, keys);
 
     // 4 stable passes of 8 bits: keys -> alt -> keys -> alt -> keys
     const int nb = L.sort_blocks;
@@ -489,6 +482,157 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
             hipLaunchKernelGGL(refit_top_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, sorted, n, children,
                                parents, ranges, boxes, flags, nodes, leaf_size);
     }
+    return hipGetLastError();
+}
+
+
+hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
+                       hipStream_t stream) {
+    const IndexLayout L = make_index_layout(n);
+    char *base = (char *)ws;
+    IndexHeader *hdr = (IndexHeader *)(base + L.header);
+    uint32_t *keys = (uint32_t *)(base + L.keys);
+    float *reduce = (float *)(base + L.reduce);
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(L.reduce_blocks), dim3(kReduceThreads), 0, stream, points, n, reduce);
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kReduceThreads), 0, stream, reduce, L.reduce_blocks, hdr, n,
+                       weights ? 1 : 0, leaf_size);
+    hipLaunchKernelGGL(morton_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, hdr, keys);
+
+    return build_tail(n, points, weights, leaf_size, base, L, stream);
+}
+
+// ------------------------------------------------------------------ batches
+// Index over a batch of clouds (contiguous input ranges cloud_offsets[c] ..
+// cloud_offsets[c + 1]): per-cloud AABB and clipping box, keys cloud id ||
+// per-cloud Morton code, one sort and hierarchy for all points.  A cloud's
+// points then occupy its own input range in sorted order, under one subtree.
+
+__device__ __forceinline__ int cloud_of(const int64_t *__restrict__ off, int n_clouds, int64_t i) {
+    int lo = 0, hi = n_clouds - 1;  // largest c with off[c] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+    batch_bbox_kernel(const float *__restrict__ pts, const int64_t *__restrict__ off, int n_clouds, int has_w,
+                      int leaf_size, IndexHeader *__restrict__ clouds, IndexHeader *__restrict__ hdr, int64_t n) {
+    const int c = blockIdx.x;
+    const int64_t a = off[c], b = off[c + 1];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            const float v = pts[3 * i + k];
+            mn[k] = fminf(mn[k], v);
+            mx[k] = fmaxf(mx[k], v);
+        }
+    }
+    __shared__ float red[6][kReduceThreads];
+    for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = mn[k];
+        red[3 + k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fminf(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+                red[3 + k][threadIdx.x] = fmaxf(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        IndexHeader h;
+        for (int k = 0; k < 3; ++k) {
+            const float lo = b > a ? red[k][0] : 0.0f, hi = b > a ? red[3 + k][0] : 0.0f;
+            h.raw_lo[k] = lo;
+            h.raw_hi[k] = hi;
+            // the single-cloud box, in the same FP32 order (voronoi_ultra.cu:237-238,
+            // voronoi_convex_cell.cuh:187-193)
+            h.box_lo[k] = __fsub_rn(__fsub_rn(lo, 1.0f), 1e-5f);
+            h.box_hi[k] = __fadd_rn(__fadd_rn(hi, 1.0f), 1e-5f);
+        }
+        h.root_ref = 0xFFFFFFFFu;  // set after the hierarchy is built
+        h.n = (uint32_t)(b - a);
+        h.has_weights = (uint32_t)has_w;
+        h.leaf_size = (uint32_t)leaf_size;
+        h.has_duplicates = 0u;
+        clouds[c] = h;
+        if (c == 0) {
+            h.n = (uint32_t)n;
+            hdr[0] = h;  // batch-wide fields (duplicates flag, sizes); its box is unused
+        }
+    }
+}
+
+__global__ void batch_morton_kernel(const float *__restrict__ pts, int64_t n, const int64_t *__restrict__ off,
+                                    int n_clouds, const IndexHeader *__restrict__ clouds, int cloud_bits,
+                                    int axis_bits, uint32_t *__restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cloud_of(off, n_clouds, i);
+    const IndexHeader &h = clouds[c];
+    const float ex = fmaxf(fmaxf(h.raw_hi[0] - h.raw_lo[0], h.raw_hi[1] - h.raw_lo[1]), h.raw_hi[2] - h.raw_lo[2]);
+    const float cells = (float)(1u << axis_bits);
+    const float scale = ex > 0.0f ? cells / ex : 0.0f;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float t = (pts[3 * i + k] - h.raw_lo[k]) * scale;
+        t = fminf(fmaxf(t, 0.0f), cells - 1.0f);
+        q[k] = (uint32_t)t;
+    }
+    const uint32_t m = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    keys[i] = cloud_bits > 0 ? (((uint32_t)c << (32 - cloud_bits)) | m) : m;
+}
+
+// Root of each cloud's subtree: the node whose range is exactly the cloud's
+// sorted range (it exists: the cloud id is the keys' leading bits).
+__global__ void batch_roots_kernel(int64_t n, const int64_t *__restrict__ off, int n_clouds,
+                                   const uint32_t *__restrict__ parents, const uint2 *__restrict__ ranges,
+                                   int leaf_size, const IndexHeader *__restrict__ hdr,
+                                   IndexHeader *__restrict__ clouds) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_clouds) return;
+    const int64_t a = off[c], cnt = off[c + 1] - a;
+    uint32_t root = 0xFFFFFFFFu;
+    if (cnt > 0 && cnt <= leaf_size) {
+        root = leaf_ref((uint32_t)a, (uint32_t)cnt);
+    } else if (cnt > 0) {
+        uint32_t p = parents[(n - 1) + a];
+        root = 0u;  // never kept: the walk meets the cloud's node (a valid node either way)
+        for (int level = 0; level < 128 && p != 0xFFFFFFFFu; ++level) {
+            const uint2 r = ranges[p];
+            if ((int64_t)r.x == a && (int64_t)r.y == a + cnt - 1) {
+                root = p;
+                break;
+            }
+            p = parents[p];
+        }
+    }
+    clouds[c].root_ref = root;
+    clouds[c].has_duplicates = hdr->has_duplicates;
+}
+
+hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
+                             const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream) {
+    const BatchLayout B = make_batch_layout(n, n_clouds);
+    const IndexLayout &L = B.idx;
+    char *base = (char *)ws;
+    IndexHeader *hdr = (IndexHeader *)(base + L.header);
+    IndexHeader *clouds = (IndexHeader *)(base + B.clouds);
+    uint32_t *keys = (uint32_t *)(base + L.keys);
+    hipLaunchKernelGGL(batch_bbox_kernel, dim3(n_clouds), dim3(kReduceThreads), 0, stream, points, cloud_offsets,
+                       n_clouds, weights ? 1 : 0, leaf_size, clouds, hdr, n);
+    hipLaunchKernelGGL(batch_morton_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, cloud_offsets,
+                       n_clouds, clouds, batch_cloud_bits(n_clouds), batch_axis_bits(n_clouds), keys);
+    if (hipError_t e = build_tail(n, points, weights, leaf_size, base, L, stream)) return e;
+    hipLaunchKernelGGL(batch_roots_kernel, dim3(grid_for(n_clouds, 256)), dim3(256), 0, stream, n, cloud_offsets,
+                       n_clouds, (const uint32_t *)(base + L.parents), (const uint2 *)(base + L.ranges), leaf_size, hdr,
+                       clouds);
     return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@ hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint
 
 hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
                        hipStream_t stream);
+hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
+                             const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream);
 
 struct CellArgs {
     int64_t n;
@@ -35,6 +37,10 @@ struct CellArgs {
     int64_t big_cap;              // tier-1 queue capacity
     int64_t huge_cap;             // tier-2 queue capacity
     uint32_t *cycles;             // optional per-cell cost counters
+    const IndexHeader *clouds = nullptr;  // batch: per-cloud headers (null: one cloud)
+    const int64_t *cloud_cells = nullptr; // batch: [n_clouds + 1] sorted-cell offsets
+    const int64_t *quads = nullptr;       // batch: [n_clouds + 1] quad offsets within [begin, end)
+    int n_clouds = 0;
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
 uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells);

@@ -177,3 +177,50 @@ def test_library_argument_errors_without_gpu():
     assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, 16, None, None, None, None, None, None, 0, 1,
                                     None, None) == 1
     assert b"out of bounds" in lib.lib.pg_last_error()
+
+
+# ---------------------------------------------------------------- batches
+
+def test_batch_quads_host_helper():
+    from paragram_amd import _engine
+
+    if not os.path.exists(_engine.LIB_PATH):
+        pytest.skip("libparagram_hip.so not built")
+    lib = _engine.HipLibrary()
+    off = np.array([0, 0, 5, 6, 14, 14], np.int64)  # sizes 0, 5, 1, 8, 0
+    q = np.zeros(6, np.int64)
+    p = ctypes.c_void_p
+    assert lib.lib.pg_batch_quads(5, p(off.ctypes.data), 0, 14, p(q.ctypes.data)) == 0
+    assert list(q) == [0, 0, 2, 3, 5, 5]
+    # a sub-range cutting clouds: cells [3, 9) -> cloud 1 has 2, cloud 2 has 1, cloud 3 has 3
+    assert lib.lib.pg_batch_quads(5, p(off.ctypes.data), 3, 9, p(q.ctypes.data)) == 0
+    assert list(q) == [0, 0, 1, 2, 3, 3]
+    assert lib.lib.pg_batch_quads(5, p(off.ctypes.data), 3, 15, p(q.ctypes.data)) == 1
+    assert b"out of bounds" in lib.lib.pg_last_error()
+
+
+def test_batch_offsets_rejected_before_any_device_work():
+    from paragram_amd import _engine
+
+    if not os.path.exists(_engine.LIB_PATH):
+        pytest.skip("libparagram_hip.so not built")
+    lib = _engine.HipLibrary()
+    p = ctypes.c_void_p
+    bad = np.array([0, 5, 3, 10], np.int64)
+    assert lib.lib.pg_build_batch_index(10, 3, None, None, p(bad.ctypes.data), -1, None, 0, None) == 1
+    assert b"non-decreasing" in lib.lib.pg_last_error()
+    bad = np.array([0, 5, 9], np.int64)
+    assert lib.lib.pg_build_batch_index(10, 2, None, None, p(bad.ctypes.data), -1, None, 0, None) == 1
+    assert b"end at n" in lib.lib.pg_last_error()
+    out = ctypes.c_uint64(0)
+    assert lib.lib.pg_batch_workspace_bytes(10, 5000, ctypes.byref(out)) == 1
+
+
+def test_batch_api_validation():
+    with pytest.raises(ValueError, match="start at 0, end at N"):
+        _api._validate_cloud_offsets([0, 3, 2, 7], 7)
+    with pytest.raises(ValueError, match="start at 0, end at N"):
+        _api._validate_cloud_offsets([1, 7], 7)
+    with pytest.raises(ValueError, match="1-D integer"):
+        _api._validate_cloud_offsets([0.0, 7.0], 7)
+    assert list(_api._validate_cloud_offsets(torch.tensor([0, 2, 7]), 7)) == [0, 2, 7]
