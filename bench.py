@@ -38,6 +38,8 @@ WORKLOADS = {
            "(sigma 0.02, clamped) + 10% uniform, weights U[0, 3e-5], seed {seed}"),
     "c5": ("learned_power", 10_000_000, 4, "power diagram, {n:,} points fp32: 50% uniform + 50% from 256 "
            "Gaussians (sigma 0.01), weights U[0, 5e-6], seed {seed} (plain form)"),
+    "c4": ("lidar_batch", 256, 3, "batched Voronoi, {n} LiDAR-like clouds (log-uniform 20k-200k points, "
+           "{total:,} points in all), seed {seed}, one launch"),
 }
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
 CELL_KERNELS = ("cell_group_kernel", "cell_group_exact_kernel", "cell_group_big_kernel", "cell_group_huge_kernel")
@@ -147,14 +149,25 @@ def main() -> int:
     if _backend._C is None:
         raise RuntimeError("HIP library not built: " + str(_backend._load_error))
     dev = torch.device("cuda:0")
-    made = getattr(workloads, gen)(args.n, args.seed)
+    clouds = None
+    if gen == "lidar_batch":
+        cl = workloads.lidar_batch(args.seed, n_clouds=args.n)
+        clouds = np.concatenate([[0], np.cumsum([c.shape[0] for c in cl])]).astype(np.int64)
+        made = np.concatenate(cl)
+        del cl
+    else:
+        made = getattr(workloads, gen)(args.n, args.seed)
     pts_np, w_np = made if isinstance(made, tuple) else (made, None)
+    n_cells = pts_np.shape[0]
     pts = torch.from_numpy(pts_np).to(dev)
     w = None if w_np is None else torch.from_numpy(w_np).to(dev).reshape(-1, 1)
     lib = _backend._C.lib
 
     def step(instr=None):
-        return _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr)
+        out = _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr, clouds=clouds)
+        if clouds is not None:  # the API's per-cloud results (cloud-local ids)
+            _engine.split_batch(*out, clouds)
+        return out
 
     for _ in range(args.warmup):
         step()
@@ -166,7 +179,15 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
-    value = args.n * args.steps / elapsed / 1e6
+    value = n_cells * args.steps / elapsed / 1e6
+    per_cloud_ms = None
+    if clouds is not None:  # the same clouds as one call each, for comparison
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for c in range(clouds.shape[0] - 1):
+            _engine.build_diagram(lib, pts[clouds[c]:clouds[c + 1]], None, 8, False, None, None, -1)
+        torch.cuda.synchronize()
+        per_cloud_ms = (time.perf_counter() - tp) * 1e3
 
     if args.copy_mib > 0:  # a plain copy of known size for the PMC byte counters
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device=dev)
@@ -177,11 +198,11 @@ def main() -> int:
 
     adj, off, status = (t.cpu().numpy() for t in out)
     e_total = int(off[-1])
-    f_bar = e_total / args.n
+    f_bar = e_total / n_cells
     b_cell = 24.0 + 20.0 * f_bar
     kern_ms = [a.elapsed_time(b) for a, b in instr.cell_events]
     kern_s = float(np.mean(kern_ms)) / 1e3
-    achieved = args.n * b_cell / kern_s / 1e9
+    achieved = n_cells * b_cell / kern_s / 1e9
     trec = _traffic_record(wkey)
     traffic = trec["bytes_per_launch"] if trec else None
 
@@ -191,7 +212,9 @@ def main() -> int:
         import oracle
 
         cores = len(os.sched_getaffinity(0))
-        sample = workloads.oracle_sample(args.n, args.seed, args.cpu_sample)
+        if clouds is not None:  # the oracle on cells of the first cloud
+            pts_np, w_np = pts_np[clouds[0]:clouds[1]], None
+        sample = workloads.oracle_sample(pts_np.shape[0], args.seed, args.cpu_sample)
         # the oracle builds its k-d tree per call: time that separately
         tb = time.perf_counter()
         oracle.cells(pts_np, w_np, sample[:1], nthreads=cores)
@@ -201,11 +224,11 @@ def main() -> int:
         t_cells = time.perf_counter() - tc - t_build
         cpu = {"value": round(sample.shape[0] / max(t_cells, 1e-9) / 1e6, 4), "unit": "Mcells/s",
                "cores": cores, "kind": "port",
-               "sample": f"{sample.shape[0]} seeded cells of the same {args.n}-point cloud; CPU oracle "
+               "sample": f"{sample.shape[0]} seeded cells of the same {pts_np.shape[0]}-point cloud; CPU oracle "
                          "(FP64 filtered predicates, Shewchuk-expansion exact fallback), OpenMP; k-d tree build "
                          f"({t_build:.2f}s) excluded"}
         ok = 0
-        for k, i in enumerate(sample):
+        for k, i in enumerate(sample):  # (cloud 0 of a batch: its ids are the global ids)
             g = adj[off[i]:off[i + 1]]
             ok += int(status[i] == ref.status[k] and np.array_equal(g, ref.row(k)))
         exact = {"checked": int(sample.shape[0]), "exact": ok, "frac": ok / sample.shape[0]}
@@ -223,7 +246,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc.format(n=args.n, seed=args.seed), "form": "plain", "n_points": args.n,
+        "config": {"workload": desc.format(n=args.n, seed=args.seed, total=n_cells),
+                   "form": "batched" if clouds is not None else "plain", "n_points": n_cells,
                    "edges": e_total, "mean_faces": round(f_bar, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
@@ -233,6 +257,7 @@ def main() -> int:
                      "bytes_per_cell": round(b_cell, 2)},
         "cpu_baseline": cpu,
         "exact_cells": exact,
+        "per_cloud_calls_ms": None if per_cloud_ms is None else round(per_cloud_ms, 3),
         "stats": instr.stats,
     }
     print(json.dumps(line), flush=True)
