@@ -115,6 +115,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=None, help="override the workload's seed")
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="cells timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=("plain", "learned"), default="plain",
+                    help="learned: start each cell from the committed radius network's prediction")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed steps, one device copy of this many MiB (PMC calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -163,8 +165,15 @@ def main() -> int:
     w = None if w_np is None else torch.from_numpy(w_np).to(dev).reshape(-1, 1)
     lib = _backend._C.lib
 
+    model = None
+    if args.form == "learned":
+        from paragram_amd import learned
+
+        model = learned.load()
+
     def step(instr=None):
-        out = _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr, clouds=clouds)
+        out = _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr, clouds=clouds,
+                                    radius_model=model)
         if clouds is not None:  # the API's per-cloud results (cloud-local ids)
             _engine.split_batch(*out, clouds)
         return out
@@ -247,7 +256,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc.format(n=args.n, seed=args.seed, total=n_cells),
-                   "form": "batched" if clouds is not None else "plain", "n_points": n_cells,
+                   "form": "batched" if clouds is not None else args.form, "n_points": n_cells,
                    "edges": e_total, "mean_faces": round(f_bar, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -68,6 +68,13 @@ int pg_index_workspace_bytes(int64_t n, uint64_t *bytes);
 int pg_build_index(int64_t n, const float *points, const float *weights, int32_t leaf_size, void *workspace,
                    uint64_t workspace_bytes, void *stream);
 
+/* The k nearest neighbours (1 <= k <= 16) of every point over the index of
+ * pg_build_index: ids[i*k + t] (original ids, ascending squared distance,
+ * the point itself excluded, a short list padded with its last id or 0) and,
+ * if d2 is non-null, the squared distances (FLT_MAX for padding).
+ * Replaces cuBQL_knn_kernel (pwr_bvh.cu:159-215). */
+int pg_knn(int64_t n, const void *workspace, int32_t k, int32_t *ids, float *d2, void *stream);
+
 /* Persistent waves used by pg_compute_cells on the current device. */
 int pg_default_waves(int32_t *waves);
 
@@ -100,6 +107,53 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream);
 
+/* pg_compute_cells with two optional per-cell arrays (original order):
+ * init_half_width (input, may be null): a predicted half-width h of each cell
+ *   (max-norm distance of its vertices from the seed).  The cell then starts
+ *   as the box cut by the cube of half-width h about the seed instead of the
+ *   box; a cell whose final vertices touch the cube, or that the cube
+ *   empties, is rebuilt from the box, so the output never depends on h
+ *   (only the time does).
+ * extent (output, may be null): the half-width of each built cell (0 for an
+ *   empty cell), the training target of such predictions.
+ * No counterpart in the reference; for the learned radius, which replaces the
+ * kNN pre-clip radius of voronoi_ultra.cu:70-85 as the traversal's start. */
+int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                        const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window,
+                        const float *init_half_width, float *extent, int32_t *rows, int64_t *row_offsets,
+                        int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                        int32_t waves, uint32_t *cell_cycles, void *stream);
+
+/* ---- Learned cell radius -------------------------------------------------
+ * No counterpart in the reference (its traversal starts from the radius of a
+ * kNN pre-clip, voronoi_ultra.cu:70-85).  A 16 -> 64 -> 64 -> 2 ReLU network
+ * (5378 fp32 parameters: W1[16][64], b1, W2[64][64], b2, W3[64][2], b3;
+ * BF16 products, FP32 accumulation) predicts log(h / d_8) and faces / 16 of
+ * each cell from its 8 nearest neighbours; h feeds pg_compute_cells_ex.
+ * Training and inference run on the GPU (inference on the matrix cores). */
+int pg_mlp_param_count(void);
+
+/* The 16 features per point (original order) from pg_knn's ids and d2 (k = 8)
+ * over the index of pg_build_index; log_d8[i] = log of the 8th distance. */
+int pg_mlp_features(int64_t n, const void *workspace, const int32_t *ids, const float *d2, float *features,
+                    float *log_d8, void *stream);
+
+/* out[i*2 + 0..1] = network(features[i*16 .. i*16 + 15]). */
+int pg_mlp_forward(int64_t n, const float *features, const float *params, float *out, void *stream);
+
+/* Elements of the float scratch of pg_mlp_train_step for a batch of n rows. */
+int pg_mlp_train_scratch_elems(int64_t n, uint64_t *elems);
+
+/* One training step on n rows: loss = mean over rows with mask > 0 (count
+ * of them passed as `count`) of the squared error of both outputs against
+ * targets[i*2 + 0..1]; grad[0..5377] receives the mean gradient and
+ * grad[5378] the mean loss; with lr > 0, Adam (0.9, 0.999, 1e-8; step from 1)
+ * updates params in place.  Gradients are reduced in a fixed order, so a step
+ * is deterministic. */
+int pg_mlp_train_step(int64_t n, const float *features, const float *targets, const float *mask, float count,
+                      float *params, float *grad, float *adam_m, float *adam_v, float *scratch, float lr, int32_t step,
+                      void *stream);
+
 /* ---- Batches of clouds --------------------------------------------------
  * No counterpart in the reference, which builds one cloud per call
  * (voronoi_ultra.cu:143-376, laguerre_ultra.cu:176-454); for callers with many
@@ -128,11 +182,19 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
 
 /* pg_compute_cells over the cells [begin, end) of a batch (any range of the
  * batch, so the chunk plan applies unchanged); the persistent waves take one
- * quad per round from a global counter. */
+ * quad per round from a global counter.  guesses / guess_offsets: optional,
+ * as for pg_compute_cells (original ids of the same cloud). */
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
-                           int64_t begin, int64_t end, int32_t *rows, int64_t *row_offsets, int32_t *counts,
-                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
-                           uint32_t *cell_cycles, void *stream);
+                           int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
+                           int32_t *rows, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
+                           void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles,
+                           void *stream);
+
+/* pg_knn over a batch index: each point's neighbours come from its own cloud
+ * (the batch's ids are global); a point alone in its cloud gets its own id as
+ * padding (the cell kernels skip it). */
+int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
+                 void *stream);
 
 /* Bytes of temporary storage for pg_scan_counts over n values. */
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes);

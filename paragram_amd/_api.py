@@ -161,18 +161,18 @@ def _validate_cloud_offsets(cloud_offsets, num_points: int) -> "np.ndarray":
     return off
 
 
-def _batch(points, weights, cloud_offsets, debug, leaf_size):
+def _batch(points, weights, cloud_offsets, knn, debug, leaf_size):
     from ._backend import _C, _load_error
     from ._engine import build_diagram, split_batch
 
     if _C is None:
         raise ImportError(f"paragram_amd HIP engine unavailable: {_load_error}")
     off = _validate_cloud_offsets(cloud_offsets, points.shape[0])
-    adjacency, offsets, status = build_diagram(_C.lib, points, weights, 0, debug, None, None, leaf_size, clouds=off)
+    adjacency, offsets, status = build_diagram(_C.lib, points, weights, knn, debug, None, None, leaf_size, clouds=off)
     return [Diagram(adjacency=a, offsets=o, status=s) for a, o, s in split_batch(adjacency, offsets, status, off)]
 
 
-def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets, *, debug: bool = False,
+def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets, *, knn: int = 8, debug: bool = False,
                             leaf_size: int = -1) -> list:
     """Voronoi diagrams of a batch of clouds in one launch.
 
@@ -182,12 +182,12 @@ def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets, *, debug: bool 
     ids, each cloud clipped to its own padded box).
     """
     points = _validate_points(points)
-    return _batch(points, None, cloud_offsets, debug, leaf_size)
+    return _batch(points, None, cloud_offsets, knn, debug, leaf_size)
 
 
-def batched_power_diagram(points: torch.Tensor, weights: torch.Tensor, cloud_offsets, *, debug: bool = False,
-                          leaf_size: int = -1) -> list:
+def batched_power_diagram(points: torch.Tensor, weights: torch.Tensor, cloud_offsets, *, knn: int = 8,
+                          debug: bool = False, leaf_size: int = -1) -> list:
     """Power diagrams of a batch of weighted clouds (see batched_voronoi_diagram)."""
     points = _validate_points(points)
     weights = _validate_weights(weights, points.shape[0], points.device)
-    return _batch(points, weights, cloud_offsets, debug, leaf_size)
+    return _batch(points, weights, cloud_offsets, knn, debug, leaf_size)

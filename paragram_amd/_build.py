@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libparagram_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["pg_abi.hip", "pg_index.hip", "pg_cell.hip", "pg_csr.hip"]
+SOURCES = ["pg_abi.hip", "pg_index.hip", "pg_cell.hip", "pg_csr.hip", "pg_knn.hip", "pg_mlp.hip"]
 ARCH = "gfx950"
 
 

@@ -26,6 +26,7 @@ STAT_NAMES = {
     11: "skipped", 15: "edges",
 }
 CELL_BLOCK = 64  # cells per admission block (one wavefront)
+KNN_PRECLIP = 16  # nearest neighbours clipped first when knn > 0
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libparagram_hip.so")
@@ -58,11 +59,21 @@ class HipLibrary:
             "pg_emit_csr": ([i64, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
             "pg_symmetrize_union": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], ctypes.c_int),
             "pg_fill_u32": ([vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
+            "pg_knn": ([i64, vp, i32, vp, vp, vp], ctypes.c_int),
+            "pg_knn_batch": ([i64, vp, i32, i32, vp, vp, vp], ctypes.c_int),
+            "pg_mlp_param_count": ([], ctypes.c_int),
+            "pg_mlp_features": ([i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_mlp_forward": ([i64, vp, vp, vp, vp], ctypes.c_int),
+            "pg_mlp_train_scratch_elems": ([i64, u64p], ctypes.c_int),
+            "pg_mlp_train_step": ([i64, vp, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp, ctypes.c_float, i32, vp],
+                                  ctypes.c_int),
+            "pg_compute_cells_ex": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                     ctypes.c_uint64, i32, vp, vp], ctypes.c_int),
             "pg_batch_workspace_bytes": ([i64, i32, u64p], ctypes.c_int),
             "pg_build_batch_index": ([i64, i32, vp, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
             "pg_batch_quads": ([i32, vp, i64, i64, vp], ctypes.c_int),
-            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp,
-                                        vp], ctypes.c_int),
+            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_uint64,
+                                        i32, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -99,12 +110,18 @@ class Instrument:
 
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
                   initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
-                  instrument: Instrument | None = None, clouds=None):
+                  instrument: Instrument | None = None, clouds=None, init_h=None, extent=None,
+                  radius_model=None):
     """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device.
 
     ``clouds``: optional host int64 array of cloud offsets (a batch of clouds,
     contiguous ranges of ``points``); the result is then the batch as one
     diagram over all points, neighbour ids global.
+    ``init_h``: optional float32 [N] predicted half-width of each cell (the
+    output does not depend on it); ``extent``: optional float32 [N] that
+    receives the built half-width of each cell.  ``radius_model``: a
+    ``learned.RadiusModel`` whose prediction (kNN, features and network on
+    this call's index) supplies ``init_h``.
     """
     n = int(points.shape[0])
     if clouds is not None:
@@ -132,6 +149,10 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
             lib.call("pg_build_batch_index", n, n_clouds, _p(points), _p(weights), c_off, int(leaf_size), _p(ws),
                      ws_bytes, sp)
+        if radius_model is not None and clouds is None:
+            from . import learned
+
+            init_h = learned.predict_on_index(lib, radius_model, ws, n, dev, sp)
         if debug:
             torch.cuda.synchronize(dev)
             print("BVH Creation time: %f ms" % ((time.perf_counter() - t0) * 1e3))
@@ -148,13 +169,21 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
 
         guesses = initial_guesses
         goffs = initial_guesses_offsets
-        # The reference pre-clips each cell by its knn nearest neighbours
-        # (voronoi_ultra.cu:70-85).  Here the best-first traversal already
-        # offers points nearest-first, and a Morton-window pre-clip measured
-        # slower on the 1M uniform cloud (more clips undone later), so knn
-        # selects nothing: window 0.
         seed_window = 0
-        if guesses is not None:
+        if guesses is None and knn > 0:
+            # The reference pre-clips each cell by its knn nearest neighbours
+            # (voronoi_ultra.cu:70-85, guesses replace it, :158-169).  Here
+            # the 16 nearest (the largest count the kNN pass keeps) measured
+            # fastest on the 1M uniform and 2M power clouds.
+            k = KNN_PRECLIP
+            guesses = torch.empty(n * k, **i32)
+            if clouds is None:
+                lib.call("pg_knn", n, _p(ws), k, _p(guesses), None, sp)
+            else:
+                lib.call("pg_knn_batch", n, _p(ws), n_clouds, k, _p(guesses), None, sp)
+            goffs = torch.arange(0, n * k + 1, k, **i32)
+            initial_guesses = guesses
+        if initial_guesses is not None and initial_guesses_offsets is not None:
             # the kernel indexes guesses through the offsets: reject malformed CSR
             ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= guesses.numel())).item())
             if not ok:
@@ -179,12 +208,13 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
                 ev[0].record(stream)
             cyc = _p(instrument.cycles if instrument is not None else None)
             if clouds is None:
-                lib.call("pg_compute_cells", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, seed_window,
+                lib.call("pg_compute_cells_ex", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, seed_window,
+                         _p(init_h), _p(extent), _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats),
+                         _p(scratch), scratch_bytes, waves, cyc, sp)
+            else:
+                lib.call("pg_compute_cells_batch", n, _p(ws), n_clouds, c_off, begin, end, _p(guesses), _p(goffs),
                          _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes,
                          waves, cyc, sp)
-            else:
-                lib.call("pg_compute_cells_batch", n, _p(ws), n_clouds, c_off, begin, end, _p(rows), _p(row_offsets),
-                         _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
             if ev is not None:
                 ev[1].record(stream)
                 instrument.cell_events.append(ev)
@@ -236,6 +266,24 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
             instrument.plan = plan
         return adjacency, offsets, status
+
+
+def knn(lib: HipLibrary, points: torch.Tensor, k: int = 8, leaf_size: int = -1):
+    """``(ids int32 [N, k], d2 float32 [N, k])``: the k nearest neighbours of each
+    point (itself excluded), as the reference's kNN pass (pwr_bvh.cu:159-215)."""
+    n = int(points.shape[0])
+    dev = points.device
+    ids = torch.empty((n, k), dtype=torch.int32, device=dev)
+    d2 = torch.empty((n, k), dtype=torch.float32, device=dev)
+    if n == 0:
+        return ids, d2
+    with torch.cuda.device(dev):
+        sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        ws_bytes = lib.u64("pg_index_workspace_bytes", n)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        lib.call("pg_build_index", n, _p(points), None, int(leaf_size), _p(ws), ws_bytes, sp)
+        lib.call("pg_knn", n, _p(ws), int(k), _p(ids), _p(d2), sp)
+    return ids, d2
 
 
 def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds):

@@ -114,6 +114,60 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
     return 0;
 }
 
+int pg_knn(int64_t n, const void *workspace, int32_t k, int32_t *ids, float *d2, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
+    if (n == 0) return 0;
+    if (!workspace || !ids) return fail(1, "workspace and ids must be non-null");
+    return check_hip(pg::knn(n, workspace, k, ids, d2, as_stream(stream)), "knn");
+}
+
+int pg_mlp_param_count(void) { return 5378; }
+
+int pg_mlp_features(int64_t n, const void *workspace, const int32_t *ids, const float *d2, float *features,
+                    float *log_d8, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n == 0) return 0;
+    if (!workspace || !ids || !d2 || !features || !log_d8)
+        return fail(1, "workspace, ids, d2, features and log_d8 must be non-null");
+    return check_hip(pg::mlp_features(n, workspace, ids, d2, features, log_d8, as_stream(stream)), "mlp_features");
+}
+
+int pg_mlp_forward(int64_t n, const float *features, const float *params, float *out, void *stream) {
+    if (n < 0) return fail(1, "number of rows must be non-negative");
+    if (n == 0) return 0;
+    if (!features || !params || !out) return fail(1, "features, params and out must be non-null");
+    return check_hip(pg::mlp_forward(n, features, params, out, as_stream(stream)), "mlp_forward");
+}
+
+int pg_mlp_train_scratch_elems(int64_t n, uint64_t *elems) {
+    if (n < 0 || !elems) return fail(1, "invalid argument");
+    *elems = (uint64_t)pg::mlp_train_blocks(n) * (5378 + 1);
+    return 0;
+}
+
+int pg_mlp_train_step(int64_t n, const float *features, const float *targets, const float *mask, float count,
+                      float *params, float *grad, float *adam_m, float *adam_v, float *scratch, float lr, int32_t step,
+                      void *stream) {
+    if (n <= 0) return fail(1, "a training batch needs at least one row");
+    if (!features || !targets || !mask || !params || !grad || !adam_m || !adam_v || !scratch)
+        return fail(1, "all arrays must be non-null");
+    if (step < 1) return fail(1, "step counts from 1");
+    return check_hip(pg::mlp_train_step(n, features, targets, mask, count, params, grad, adam_m, adam_v, scratch, lr,
+                                        step, as_stream(stream)),
+                     "mlp_train_step");
+}
+
+int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
+                 void *stream) {
+    if (int r = check_n(n)) return r;
+    if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (n == 0) return 0;
+    if (!workspace || !ids) return fail(1, "workspace and ids must be non-null");
+    return check_hip(pg::knn(n, workspace, k, ids, d2, as_stream(stream), n_clouds), "knn_batch");
+}
+
 int pg_default_waves(int32_t *waves) {
     if (!waves) return fail(1, "waves is null");
     int dev = 0;
@@ -146,6 +200,16 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream) {
+    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, seed_window, nullptr,
+                               nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes, waves,
+                               cell_cycles, stream);
+}
+
+int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                        const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window,
+                        const float *init_half_width, float *extent, int32_t *rows, int64_t *row_offsets,
+                        int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                        int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
     if (begin == end) return 0;
@@ -181,13 +245,16 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
     a.big_cap = big_cap;
     a.huge_cap = huge_cap;
     a.cycles = cell_cycles;
+    a.init_h = init_half_width;
+    a.extent = extent;
     return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells");
 }
 
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
-                           int64_t begin, int64_t end, int32_t *rows, int64_t *row_offsets, int32_t *counts,
-                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
-                           uint32_t *cell_cycles, void *stream) {
+                           int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
+                           int32_t *rows, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
+                           void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles,
+                           void *stream) {
     if (int r = check_n(n)) return r;
     if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
@@ -214,8 +281,10 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.index_ws = workspace;
     a.begin = begin;
     a.end = end;
-    a.guesses = nullptr;
-    a.guess_offsets = nullptr;
+    if ((guesses == nullptr) != (guess_offsets == nullptr))
+        return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
+    a.guesses = guesses;
+    a.guess_offsets = guess_offsets;
     a.guesses_sorted = 0;
     a.seed_window = 0;
     a.rows = rows;

@@ -71,6 +71,8 @@ struct CellParams {
     const int64_t *cloud_cells;   // [n_clouds + 1] sorted-cell range of each cloud
     const int64_t *quads;         // [n_clouds + 1] first work quad of each cloud within [begin, end)
     int n_clouds;
+    const float *init_h;          // optional [n] (original order): predicted half-width of each cell
+    float *extent;                // optional [n] (original order): half-width of each built cell
 };
 
 // A vertex is a triple of plane slots packed kBits bits apiece.
@@ -142,6 +144,8 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.cloud_cells = a.cloud_cells;
     prm.quads = a.quads;
     prm.n_clouds = a.n_clouds;
+    prm.init_h = a.init_h;
+    prm.extent = a.extent;
     char *sc = (char *)a.scratch;
     prm.counters = (uint32_t *)sc;
     prm.big_list = (uint32_t *)(sc + 256);

@@ -323,6 +323,7 @@ struct Group {
     int status;
     bool overflow;  // the cell left this tier (capacity, or `bail`)
     bool bail;      // an FP32 test was undecided and the tier has no FP64 path
+    bool cube;      // the cell started inside a predicted cube (see init_box)
     bool active;  // the group holds a cell in this round
     bool dirty;   // the cell changed since the AABB was computed
     uint64_t bloom;
@@ -873,22 +874,30 @@ struct Group {
         }
     }
 
-    __device__ __forceinline__ void init_box() {
+    // The initial cell: the box, or (cube_h > 0, a predicted half-width in
+    // doubled coordinates) the box cut by the cube |y_k| <= cube_h, whose
+    // planes carry owner codes -7..-12.  A cell whose final vertices touch a
+    // cube plane, or that empties, is rebuilt without the cube.
+    __device__ __forceinline__ void init_box(float cube_h = 0.0f) {
+        bool from_cube = false;
         if (gl < 6) {
             const int axis = gl >> 1, side = gl & 1;
             // (a lane-dependent choice among the members sx, sy, sz would be
             // compiled into a choice of addresses, moving them to scratch)
             const float4 sp = pts[seed];
             const float s = axis == 0 ? sp.x : (axis == 1 ? sp.y : sp.z);
-            const float h = side ? 2.0f * (hdr->box_hi[axis] - s) : 2.0f * (s - hdr->box_lo[axis]);
+            float h = side ? 2.0f * (hdr->box_hi[axis] - s) : 2.0f * (s - hdr->box_lo[axis]);
+            from_cube = cube_h > 0.0f && cube_h < h;
+            if (from_cube) h = cube_h;
             float4 pl = make_float4(0.f, 0.f, 0.f, h);
             const float sg = side ? 1.0f : -1.0f;
             if (axis == 0) pl.x = sg;
             if (axis == 1) pl.y = sg;
             if (axis == 2) pl.z = sg;
             S->pl[gl] = pl;
-            S->plx[gl] = make_float2(fabsf(h), __int_as_float(-1 - gl));
+            S->plx[gl] = make_float2(fabsf(h), __int_as_float((from_cube ? -7 : -1) - gl));
         }
+        cube = gany(from_cube);
         np = 6;
         // the 8 corners go to slots 0..7 (lanes 0..7 of row 0)
         if (gl < 8) {
@@ -1118,8 +1127,25 @@ struct Group {
         if (gl == 0) {
             prm.counts[orig] = F;
             prm.status[orig] = status;
+            if (prm.extent)
+                prm.extent[orig] = status == kEmptyCell ? 0.0f
+                                                       : 0.5f * fmaxf(fmaxf(fmaxf(-L_[0], U_[0]), fmaxf(-L_[1], U_[1])),
+                                                                      fmaxf(-L_[2], U_[2]));
         }
         return true;
+    }
+
+    // Does a final vertex lie on a plane of the predicted cube (group op)?
+    __device__ __forceinline__ bool cube_touched() const {
+        uint64_t used[kPW];
+        used_planes(used);
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < kPRows; ++k) {
+            const int p = gl + G * k;
+            if (p < np && mask_bit(used, p)) hit |= __float_as_int(S->plx[p].y) <= -7;
+        }
+        return gany(hit);
     }
 
     __device__ __forceinline__ void start(uint32_t cell, const CellParams &prm) {
@@ -1129,8 +1155,8 @@ struct Group {
         status = kSuccess;
         overflow = false;
         bail = false;
-        init_box();
         const uint32_t orig = perm[cell];
+        init_box((!kExact && prm.init_h) ? 2.0f * prm.init_h[orig] : 0.0f);
         in_guess = true;
         if (!prm.guesses && prm.seed_window > 0) {
             // Morton neighbours first: the cheap analogue of clipping by the
@@ -1251,6 +1277,9 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         if (!active) return;
         const uint32_t orig = prm.perm[cell];
         bool done = false;
+        // a cell built inside a predicted cube stands only if the cube cut nothing
+        if (!kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
+            g.overflow = g.bail = true;
         if (!g.overflow) {
             if (g.gl == 0) prm.row_offsets[cell] = c * prm.row_stride;
             done = g.emit(prm, orig, prm.rows + c * prm.row_stride, prm.row_stride);

@@ -15,6 +15,14 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
                        hipStream_t stream);
 hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
                              const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream);
+hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
+               int n_clouds = 0);
+hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
+                        float *logd8, hipStream_t stream);
+hipError_t mlp_forward(int64_t n, const float *X, const float *params, float *Y, hipStream_t stream);
+int mlp_train_blocks(int64_t n);
+hipError_t mlp_train_step(int64_t n, const float *X, const float *T, const float *mask, float count, float *params,
+                          float *grad, float *m, float *v, float *part, float lr, int step, hipStream_t stream);
 
 struct CellArgs {
     int64_t n;
@@ -41,6 +49,8 @@ struct CellArgs {
     const int64_t *cloud_cells = nullptr; // batch: [n_clouds + 1] sorted-cell offsets
     const int64_t *quads = nullptr;       // batch: [n_clouds + 1] quad offsets within [begin, end)
     int n_clouds = 0;
+    const float *init_h = nullptr;        // optional predicted half-width per cell (original order)
+    float *extent = nullptr;              // optional built half-width per cell (original order)
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
 uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells);

@@ -1,0 +1,143 @@
+// k nearest neighbours of every point over the LBVH (the reference's
+// cuBQL_knn_kernel, pwr_bvh.cu:159-215: the K nearest by squared distance,
+// the query's own id excluded, short lists padded with the last id found or
+// 0).  Used for the learned radius features and as initial guesses.
+//
+// One lane per query in Morton order (neighbouring lanes walk neighbouring
+// parts of the hierarchy).  Depth-first, nearer child first; the farther
+// child waits on a per-lane stack in LDS as (parent << 1 | child) and is
+// re-tested against the current K-th distance when it comes out.  A Karras
+// hierarchy over 32-bit keys and < 2^26 points is less than 64 levels deep,
+// and every level pushes at most one entry, so 64 entries never overflow.
+#include "pg_common.hpp"
+#include "pg_kernels.hpp"
+
+#include <float.h>
+
+namespace pg {
+
+constexpr int kKnnThreads = 128;
+constexpr int kKnnStack = 64;
+
+__device__ __forceinline__ float box_dist2(float4 lo, float4 hi, float qx, float qy, float qz) {
+    const float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
+    const float dy = fmaxf(fmaxf(lo.y - qy, qy - hi.y), 0.0f);
+    const float dz = fmaxf(fmaxf(lo.z - qz, qz - hi.z), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <int K>
+__global__ void __launch_bounds__(kKnnThreads)
+    knn_kernel(int64_t n, const IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
+               const uint32_t *__restrict__ perm, const Node *__restrict__ nodes, int k_out,
+               int32_t *__restrict__ out_ids, float *__restrict__ out_d2, const IndexHeader *__restrict__ clouds,
+               const int64_t *__restrict__ cloud_cells, int n_clouds) {
+    __shared__ uint32_t stack[kKnnStack][kKnnThreads];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = pts[i];
+    float d[K];
+    int32_t id[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) d[t] = FLT_MAX, id[t] = -1;
+    // a node is pruned when its box is farther than the K-th distance (with
+    // a relative margin for the rounding of the two squared distances)
+    auto far = [&](float bd) { return bd * (1.0f - 1e-6f) > d[K - 1]; };
+    // a batch searches the query's own cloud (its subtree)
+    uint32_t ref = hdr->root_ref;
+    if (clouds != nullptr) {
+        int lo = 0, hi = n_clouds - 1;  // largest c with cloud_cells[c] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cloud_cells[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        ref = clouds[lo].root_ref;
+    }
+    int sp = 0;
+    while (true) {
+        if (is_leaf(ref)) {
+            const uint32_t f = leaf_first(ref), c = leaf_count(ref);
+            for (uint32_t t = 0; t < c; ++t) {
+                const uint32_t j = f + t;
+                if ((int64_t)j == i) continue;
+                const float4 p = pts[j];
+                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                const float dd = dx * dx + dy * dy + dz * dz;
+                if (dd < d[K - 1]) {  // sorted insertion, ties keep the earlier point
+                    d[K - 1] = dd;
+                    id[K - 1] = (int32_t)j;
+#pragma unroll
+                    for (int u = K - 1; u > 0; --u) {
+                        if (d[u] < d[u - 1]) {
+                            const float td = d[u];
+                            d[u] = d[u - 1];
+                            d[u - 1] = td;
+                            const int32_t ti = id[u];
+                            id[u] = id[u - 1];
+                            id[u - 1] = ti;
+                        }
+                    }
+                }
+            }
+        } else {
+            const Node nd = nodes[ref];
+            const float b0 = box_dist2(nd.lo0, nd.hi0, q.x, q.y, q.z);
+            const float b1 = box_dist2(nd.lo1, nd.hi1, q.x, q.y, q.z);
+            const bool near0 = b0 <= b1;
+            const float bn = near0 ? b0 : b1, bf = near0 ? b1 : b0;
+            if (!far(bf) && sp < kKnnStack) stack[sp++][threadIdx.x] = (ref << 1) | (near0 ? 1u : 0u);
+            if (!far(bn)) {
+                ref = __float_as_uint(near0 ? nd.hi0.w : nd.hi1.w);
+                continue;
+            }
+        }
+        bool have = false;
+        while (sp > 0 && !have) {
+            const uint32_t e = stack[--sp][threadIdx.x];
+            const Node nd = nodes[e >> 1];
+            const float4 lo = (e & 1u) ? nd.lo1 : nd.lo0, hi = (e & 1u) ? nd.hi1 : nd.hi0;
+            if (!far(box_dist2(lo, hi, q.x, q.y, q.z))) {
+                ref = __float_as_uint(hi.w);
+                have = true;
+            }
+        }
+        if (!have) break;
+    }
+    // short lists repeat the last id found; with none found the reference
+    // writes 0, a batch writes the point itself (0 may be another cloud's)
+    const int64_t o = (int64_t)perm[i] * k_out;
+    int32_t last = clouds != nullptr ? (int32_t)perm[i] : 0;
+    for (int t = 0; t < k_out; ++t) {
+        if (id[t] >= 0) last = (int32_t)perm[id[t]];
+        out_ids[o + t] = last;
+        if (out_d2) out_d2[o + t] = id[t] >= 0 ? d[t] : FLT_MAX;
+    }
+}
+
+hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds) {
+    if (n <= 0) return hipSuccess;
+    const IndexLayout L = make_index_layout(n);
+    const char *ws = (const char *)index_ws;
+    const IndexHeader *clouds = nullptr;
+    const int64_t *cloud_cells = nullptr;
+    if (n_clouds > 0) {
+        const BatchLayout B = make_batch_layout(n, n_clouds);
+        clouds = (const IndexHeader *)(ws + B.clouds);
+        cloud_cells = (const int64_t *)(ws + B.offsets);
+    }
+    const IndexHeader *hdr = (const IndexHeader *)(ws + L.header);
+    const float4 *pts = (const float4 *)(ws + L.points);
+    const uint32_t *perm = (const uint32_t *)(ws + L.perm);
+    const Node *nodes = (const Node *)(ws + L.nodes);
+    const unsigned grid = (unsigned)((n + kKnnThreads - 1) / kKnnThreads);
+    if (k <= 8)
+        hipLaunchKernelGGL(knn_kernel<8>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k, ids,
+                           d2, clouds, cloud_cells, n_clouds);
+    else
+        hipLaunchKernelGGL(knn_kernel<16>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k,
+                           ids, d2, clouds, cloud_cells, n_clouds);
+    return hipGetLastError();
+}
+
+}  // namespace pg
