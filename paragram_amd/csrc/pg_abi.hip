@@ -168,6 +168,15 @@ int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, 
     return check_hip(pg::knn(n, workspace, k, ids, d2, as_stream(stream), n_clouds), "knn_batch");
 }
 
+int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
+    if (n_clouds < 0 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [0, 4096]");
+    if (n == 0) return 0;
+    if (!workspace || !ids) return fail(1, "workspace and ids must be non-null");
+    return check_hip(pg::knn(n, workspace, k, ids, nullptr, as_stream(stream), n_clouds, true), "knn_guesses");
+}
+
 int pg_default_waves(int32_t *waves) {
     if (!waves) return fail(1, "waves is null");
     int dev = 0;
@@ -200,14 +209,15 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream) {
-    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, seed_window, nullptr,
-                               nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes, waves,
-                               cell_cycles, stream);
+    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, 0, seed_window,
+                               nullptr, nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes,
+                               waves, cell_cycles, stream);
 }
 
 int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
-                        const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window,
-                        const float *init_half_width, float *extent, int32_t *rows, int64_t *row_offsets,
+                        const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
+                        int32_t seed_window, const float *init_half_width, float *extent, int32_t *rows,
+                        int64_t *row_offsets,
                         int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                         int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
@@ -215,8 +225,10 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     if (begin == end) return 0;
     if (!workspace || !rows || !row_offsets || !counts || !status || !stats || !scratch)
         return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
-    if ((guesses == nullptr) != (guess_offsets == nullptr))
+    if (guess_stride < 0 || guess_stride > 64) return fail(1, "guess_stride must be in [0, 64]");
+    if (guess_stride == 0 && (guesses == nullptr) != (guess_offsets == nullptr))
         return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
+    if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
     if (waves < 1) return fail(1, "waves must be positive");
     const int w0 = (waves + 3) / 4 * 4;
     const int64_t big_cap = pg::tier1_capacity(end - begin);
@@ -224,6 +236,7 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap, end - begin))
         return fail(1, "cell scratch too small");
     pg::CellArgs a;
+    a.guess_stride = guess_stride;
     a.n = n;
     a.index_ws = workspace;
     a.begin = begin;
@@ -252,9 +265,9 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
 
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
                            int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t *rows, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
-                           void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles,
-                           void *stream) {
+                           int32_t guess_stride, int32_t *rows, int64_t *row_offsets, int32_t *counts,
+                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
+                           uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
@@ -281,8 +294,11 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.index_ws = workspace;
     a.begin = begin;
     a.end = end;
-    if ((guesses == nullptr) != (guess_offsets == nullptr))
+    if (guess_stride < 0 || guess_stride > 64) return fail(1, "guess_stride must be in [0, 64]");
+    if (guess_stride == 0 && (guesses == nullptr) != (guess_offsets == nullptr))
         return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
+    if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
+    a.guess_stride = guess_stride;
     a.guesses = guesses;
     a.guess_offsets = guess_offsets;
     a.guesses_sorted = 0;

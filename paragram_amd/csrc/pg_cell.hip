@@ -50,6 +50,7 @@ struct CellParams {
     const int32_t *guesses;
     const int32_t *guess_offsets;
     int guesses_sorted;
+    int guess_stride;             // > 0: guesses are Morton-ordered rows of this many sorted ids
     int seed_window;              // Morton neighbours offered first (0: none)
     int32_t *rows;                // tier-0 slab (row_stride per cell) then tier-1 rows
     int row_stride;
@@ -132,6 +133,7 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.guesses = a.guesses;
     prm.guess_offsets = a.guess_offsets;
     prm.guesses_sorted = a.guesses_sorted;
+    prm.guess_stride = a.guess_stride;
     prm.seed_window = a.seed_window;
     prm.rows = a.rows;
     prm.row_stride = a.row_stride;

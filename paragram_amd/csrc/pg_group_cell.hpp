@@ -1169,14 +1169,18 @@ struct Group {
             }
         }
         if (prm.guesses) {
-            const int32_t g0 = prm.guess_offsets[orig], g1 = prm.guess_offsets[orig + 1];
-            for (int32_t g = g0; g < g1 && status == kSuccess && !overflow; g += G) {
+            // rows by original id (CSR), or Morton-ordered rows of guess_stride
+            // sorted ids (the library's own kNN pre-clip)
+            const bool strided = prm.guess_stride > 0;
+            const int64_t g0 = strided ? (int64_t)cell * prm.guess_stride : (int64_t)prm.guess_offsets[orig];
+            const int64_t g1 = strided ? g0 + prm.guess_stride : (int64_t)prm.guess_offsets[orig + 1];
+            for (int64_t g = g0; g < g1 && status == kSuccess && !overflow; g += G) {
                 bool has = g + gl < g1;
                 uint32_t j = 0;
                 if (has) {
                     const int32_t gid = prm.guesses[g + gl];
                     has = gid >= 0 && (int64_t)gid < prm.n;  // voronoi_ultra.cu:76-78 skips negative ids
-                    if (has) j = prm.guesses_sorted ? (uint32_t)gid : inv[gid];
+                    if (has) j = (strided || prm.guesses_sorted) ? (uint32_t)gid : inv[gid];
                     has = has && j != cell;
                 }
                 offer_batch(has, j);

@@ -16,7 +16,7 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
 hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
                              const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream);
 hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
-               int n_clouds = 0);
+               int n_clouds = 0, bool sorted_rows = false);
 hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
                         float *logd8, hipStream_t stream);
 hipError_t mlp_forward(int64_t n, const float *X, const float *params, float *Y, hipStream_t stream);
@@ -31,6 +31,7 @@ struct CellArgs {
     const int32_t *guesses;       // may be null
     const int32_t *guess_offsets; // indexed by original id (null with guesses == null)
     int guesses_sorted;           // guesses already hold sorted indices (kNN output)
+    int guess_stride = 0;         // > 0: Morton-ordered rows of this many sorted ids (offsets unused)
     int seed_window;              // Morton neighbours offered before the traversal
     int32_t *rows;                // (end - begin) x row_stride, big_cap x kTier1Row, huge_cap x kTier2Row
     int row_stride;

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(kKnnThreads)
     knn_kernel(int64_t n, const IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
                const uint32_t *__restrict__ perm, const Node *__restrict__ nodes, int k_out,
                int32_t *__restrict__ out_ids, float *__restrict__ out_d2, const IndexHeader *__restrict__ clouds,
-               const int64_t *__restrict__ cloud_cells, int n_clouds) {
+               const int64_t *__restrict__ cloud_cells, int n_clouds, int sorted_rows) {
     __shared__ uint32_t stack[kKnnStack][kKnnThreads];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -105,7 +105,17 @@ __global__ void __launch_bounds__(kKnnThreads)
         if (!have) break;
     }
     // short lists repeat the last id found; with none found the reference
-    // writes 0, a batch writes the point itself (0 may be another cloud's)
+    // writes 0, a batch writes the point itself (0 may be another cloud's).
+    // sorted_rows: row i (sorted order) of sorted ids, self as padding.
+    if (sorted_rows) {
+        const int64_t o = i * k_out;
+        int32_t last = (int32_t)i;
+        for (int t = 0; t < k_out; ++t) {
+            if (id[t] >= 0) last = id[t];
+            out_ids[o + t] = last;
+        }
+        return;
+    }
     const int64_t o = (int64_t)perm[i] * k_out;
     int32_t last = clouds != nullptr ? (int32_t)perm[i] : 0;
     for (int t = 0; t < k_out; ++t) {
@@ -115,7 +125,8 @@ __global__ void __launch_bounds__(kKnnThreads)
     }
 }
 
-hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds) {
+hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds,
+               bool sorted_rows) {
     if (n <= 0) return hipSuccess;
     const IndexLayout L = make_index_layout(n);
     const char *ws = (const char *)index_ws;
@@ -133,10 +144,10 @@ hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, 
     const unsigned grid = (unsigned)((n + kKnnThreads - 1) / kKnnThreads);
     if (k <= 8)
         hipLaunchKernelGGL(knn_kernel<8>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k, ids,
-                           d2, clouds, cloud_cells, n_clouds);
+                           d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0);
     else
         hipLaunchKernelGGL(knn_kernel<16>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k,
-                           ids, d2, clouds, cloud_cells, n_clouds);
+                           ids, d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0);
     return hipGetLastError();
 }
 
