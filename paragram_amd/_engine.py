@@ -112,7 +112,7 @@ class Instrument:
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
                   initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
                   instrument: Instrument | None = None, clouds=None, init_h=None, extent=None,
-                  radius_model=None):
+                  radius_model=None, preclip: int | None = None):
     """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device.
 
     ``clouds``: optional host int64 array of cloud offsets (a batch of clouds,
@@ -122,7 +122,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
     output does not depend on it); ``extent``: optional float32 [N] that
     receives the built half-width of each cell.  ``radius_model``: a
     ``learned.RadiusModel`` whose prediction (kNN, features and network on
-    this call's index) supplies ``init_h``.
+    this call's index) supplies ``init_h``.  ``preclip``: nearest neighbours
+    clipped first when knn > 0 (default KNN_PRECLIP, 1..16).
     """
     n = int(points.shape[0])
     if clouds is not None:
@@ -177,7 +178,7 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             # (voronoi_ultra.cu:70-85, guesses replace it, :158-169).  Here
             # the 16 nearest (the largest count the kNN pass keeps) measured
             # fastest on the 1M uniform and 2M power clouds.
-            guess_stride = KNN_PRECLIP
+            guess_stride = KNN_PRECLIP if preclip is None else int(preclip)
             guesses = torch.empty(n * guess_stride, **i32)
             lib.call("pg_knn_guesses", n, _p(ws), 0 if clouds is None else n_clouds, guess_stride, _p(guesses), sp)
             goffs = None
