@@ -35,7 +35,7 @@ namespace pg {
 
 template <int V, int P>
 struct alignas(16) GroupLDS {
-    float4 vpos[V];
+    float4 vpos[V];   // y~ and w (see store_vertex)
     float4 pl[P];
     float2 plx[P];
     uint32_t vtri[V];
@@ -52,6 +52,7 @@ struct alignas(16) CandLDS {
 
 struct Cand {
     float ax, ay, az, H, pH, pa1;
+    float c2;  // 2e-6 pH + tiny: the plane's part of the removal filter's bound
     int owner;
 };
 
@@ -319,6 +320,7 @@ struct Group {
     uint64_t valid[kW];
     int np;
     float L_[3], U_[3];  // AABB of the cell
+    float EM_;           // largest vertex error radius w of the cell
     bool bounded;
     int status;
     bool overflow;  // the cell left this tier (capacity, or `bail`)
@@ -435,9 +437,20 @@ struct Group {
         } else if (D < -eD && pD > kTiny32) {
             count(kStatInconsistent);
         }
-        S->vpos[slot] = pos;
+        store_vertex(slot, pos);
         S->vtri[slot] = tri;
         return ok;
+    }
+
+    // Vertex record: y~ and w = eps (1 + s) + 2e-6 max_k |y~_k|, which bounds
+    // both the position error eps and the removal filter's rounding term
+    // 2e-6 sum_k |a_k y~_k| <= 2e-6 |a|_1 max_k |y~_k|.  A negative w marks a
+    // vertex left to the FP64 path; +inf stays +inf.
+    __device__ __forceinline__ void store_vertex(int slot, float4 pos) {
+        float w = pos.w;
+        if (w >= 0.0f && w < FLT_MAX)
+            w = w * (1.0f + kSlack) + kG32 * fmaxf(fabsf(pos.x), fmaxf(fabsf(pos.y), fabsf(pos.z)));
+        S->vpos[slot] = make_float4(pos.x, pos.y, pos.z, w);
     }
 
     // FP64 positions for the slots of `m` that make_vertex left unresolved
@@ -450,25 +463,26 @@ struct Group {
         while (pend) {
             const int v = gl + G * (int)__builtin_ctz(pend);
             pend &= pend - 1u;
-            S->vpos[v] = fp64_position<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw);
+            store_vertex(v, fp64_position<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw));
         }
     }
 
     // Is vertex v strictly beyond the plane of q?  Filter: the error of
     // a~ . y~ - H~ against a . y - H is at most |a|_1 eps + 2e-6 (sum |a_k y_k| + pH).
     // Returns 1 (beyond), 0 (not beyond) or -1 (undecided: FP64 recheck).
+    // (with the stored w: |a|_1 w + 2e-6 pH + tiny bounds the error of the
+    // FMA chain below against a . y - H)
     __device__ __forceinline__ int beyond32(int v, const Cand &q) const {
         const float4 p = S->vpos[v];
-        const float tx = q.ax * p.x, ty = q.ay * p.y, tz = q.az * p.z;
-        const float Pv = tx + ty + tz - q.H;
-        const float err = q.pa1 * p.w * (1.0f + kSlack) + kG32 * (fabsf(tx) + fabsf(ty) + fabsf(tz) + q.pH) + kTiny32;
+        const float Pv = fmaf(q.ax, p.x, fmaf(q.ay, p.y, fmaf(q.az, p.z, -q.H)));
+        const float err = fmaf(q.pa1, p.w, q.c2);
         if (fabsf(Pv) > err) return Pv > 0.0f ? 1 : 0;
         return -1;
     }
 
     // ---------------------------------------------------------- bounds
     __device__ __forceinline__ void update_bounds() {
-        float lo0 = FLT_MAX, lo1 = FLT_MAX, lo2 = FLT_MAX, hi0 = -FLT_MAX, hi1 = -FLT_MAX, hi2 = -FLT_MAX;
+        float lo0 = FLT_MAX, lo1 = FLT_MAX, lo2 = FLT_MAX, hi0 = -FLT_MAX, hi1 = -FLT_MAX, hi2 = -FLT_MAX, em = 0.0f;
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
@@ -481,11 +495,13 @@ struct Group {
                 hi1 = fmaxf(hi1, p.y + p.w);
                 lo2 = fminf(lo2, p.z - p.w);
                 hi2 = fmaxf(hi2, p.z + p.w);
+                em = fmaxf(em, p.w);
             }
         }
         bounded = !gany(!ok);
         L_[0] = gmin(lo0), L_[1] = gmin(lo1), L_[2] = gmin(lo2);
         U_[0] = gmax(hi0), U_[1] = gmax(hi1), U_[2] = gmax(hi2);
+        EM_ = gmax(em);
     }
 
     // Can the plane of candidate c not cut the cell?  max over the AABB of
@@ -540,27 +556,30 @@ struct Group {
     }
 
     // Exact form of the prune, per vertex (group op).  Vertex y is beyond the
-    // plane of p iff a . y - |a|^2 + dw > 0, i.e. |a - c|^2 < |c|^2 + dw with
-    // c = y / 2: p lies in the ball about the vertex through the seed (grown
-    // by the weight difference).  No point of the box cuts the cell when the
-    // box misses every vertex's ball.  Each vertex's error radius eps widens
-    // the test: |c_k - c~_k| <= eps / 2.
+    // plane of p iff a . y - |a|^2 + dw > 0, i.e. in doubled coordinates
+    // |2a - y|^2 < |y|^2 + 4 dw: p lies in the ball about the vertex through
+    // the seed (grown by the weight difference).  No point of the box cuts the
+    // cell when the box misses every vertex's ball.  The box [2 al, 2 ah] is
+    // widened by the cell's largest error radius (each coordinate of y lies
+    // within w of y~), and sum_k (|y~_k| + w)^2 bounds |y|^2.
     __device__ __forceinline__ bool balls_clear(float alx, float aly, float alz, float ahx, float ahy, float ahz,
                                                 float dwmax) const {
+        const float Ax = fmaf(2.0f, alx, -EM_), Ay = fmaf(2.0f, aly, -EM_), Az = fmaf(2.0f, alz, -EM_);
+        const float Bx = fmaf(2.0f, ahx, EM_), By = fmaf(2.0f, ahy, EM_), Bz = fmaf(2.0f, ahz, EM_);
+        const float dw4 = 4.0f * dwmax;
         bool cut = false;
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
             if (!own(valid, k)) continue;
-            const float4 p = S->vpos[gl + G * k];
-            const float e = 0.5f * p.w;
-            const float cx = 0.5f * p.x, cy = 0.5f * p.y, cz = 0.5f * p.z;
-            const float ex = fmaxf(fmaxf(alx - cx, cx - ahx) - e, 0.0f);
-            const float ey = fmaxf(fmaxf(aly - cy, cy - ahy) - e, 0.0f);
-            const float ez = fmaxf(fmaxf(alz - cz, cz - ahz) - e, 0.0f);
-            const float rx = fabsf(cx) + e, ry = fabsf(cy) + e, rz = fabsf(cz) + e;
-            const float d2 = ex * ex + ey * ey + ez * ez;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            cut |= !(d2 * (1.0f - kSlack) > r2 * (1.0f + kSlack) + dwmax);
+            const int v = gl + G * k;
+            const float4 p = S->vpos[v];
+            const float rx = fabsf(p.x) + p.w, ry = fabsf(p.y) + p.w, rz = fabsf(p.z) + p.w;
+            const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) * (1.0f + 4.0f * kSlack);
+            const float tx = fmaxf(fmaxf(Ax - p.x, p.x - Bx), 0.0f);
+            const float ty = fmaxf(fmaxf(Ay - p.y, p.y - By), 0.0f);
+            const float tz = fmaxf(fmaxf(Az - p.z, p.z - Bz), 0.0f);
+            const float d2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+            cut |= !(d2 * (1.0f - kSlack) > r2 + dw4);
         }
         return !gany(cut);
     }
@@ -859,6 +878,7 @@ struct Group {
             q.H = aH.w;
             q.pH = po.x;
             q.pa1 = fabsf(q.ax) + fabsf(q.ay) + fabsf(q.az);
+            q.c2 = fmaf(kG32, q.pH, kTiny32);
             q.owner = __float_as_int(po.y);
             if (bloom && has_plane(q.owner)) continue;
             if (clip(q)) {
