@@ -320,6 +320,7 @@ struct Group {
     uint64_t valid[kW];
     int np;
     float L_[3], U_[3];  // AABB of the cell
+    float Lq_[3], Uq_[3];  // their squares (the octant corner radii)
     float EM_;           // largest vertex error radius w of the cell
     bool bounded;
     int status;
@@ -501,6 +502,8 @@ struct Group {
         bounded = !gany(!ok);
         L_[0] = gmin(lo0), L_[1] = gmin(lo1), L_[2] = gmin(lo2);
         U_[0] = gmax(hi0), U_[1] = gmax(hi1), U_[2] = gmax(hi2);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Lq_[d] = L_[d] * L_[d], Uq_[d] = U_[d] * U_[d];
         EM_ = gmax(em);
     }
 
@@ -517,14 +520,16 @@ struct Group {
     }
 
     // Node prune.  Every point p of the box has a = p - s with |a| >= d and
-    // H = |a|^2 - (w_p - w_s) >= |a|^2 - dw (dw = w_max - w_s).  Two bounds
-    // on the cell side, either of which suffices:
+    // H = |a|^2 - (w_p - w_s) >= |a|^2 - dw (dw = w_max - w_s).  Two tests,
+    // either of which suffices:
     //  * octant radius (voronoi_bvh.cuh:71-107, pwr_bvh.cuh:150-185): a . y
     //    <= |a| rho with rho the largest AABB corner radius over the octants
     //    the box overlaps, so no cut when |a|^2 - rho |a| - dw >= 0 for all
     //    |a| >= d, i.e. 2d >= rho and (d^2 - dw)^2 >= rho^2 d^2, d^2 > dw;
-    //  * support: a . y <= sum_k max over the box's faces of a_k L_k, a_k U_k
-    //    (separable per axis), against the lower bound d^2 - dw of H.
+    //  * the per-vertex balls (balls_clear), the exact form.
+    // kSplit: the two halves of the group test two boxes at once (each lane
+    // is given its half's box).
+    template <bool kSplit>
     __device__ __forceinline__ bool node_prunable(float4 lo, float4 hi, float &key) const {
         const float alx = lo.x - sx, aly = lo.y - sy, alz = lo.z - sz;
         const float ahx = hi.x - sx, ahy = hi.y - sy, ahz = hi.z - sz;
@@ -538,40 +543,39 @@ struct Group {
         if (!bounded) return false;
         // the octants a box overlaps form a product set, so the largest
         // corner radius over them is a sum of per-axis maxima
-        const float rho2 = (fmaxf(alx <= 0.0f ? L_[0] * L_[0] : 0.0f, ahx >= 0.0f ? U_[0] * U_[0] : 0.0f) +
-                            fmaxf(aly <= 0.0f ? L_[1] * L_[1] : 0.0f, ahy >= 0.0f ? U_[1] * U_[1] : 0.0f) +
-                            fmaxf(alz <= 0.0f ? L_[2] * L_[2] : 0.0f, ahz >= 0.0f ? U_[2] * U_[2] : 0.0f)) *
+        const float rho2 = (fmaxf(alx <= 0.0f ? Lq_[0] : 0.0f, ahx >= 0.0f ? Uq_[0] : 0.0f) +
+                            fmaxf(aly <= 0.0f ? Lq_[1] : 0.0f, ahy >= 0.0f ? Uq_[1] : 0.0f) +
+                            fmaxf(alz <= 0.0f ? Lq_[2] : 0.0f, ahz >= 0.0f ? Uq_[2] : 0.0f)) *
                            (1.0f + kSlack);
         const float d2lo = d2 * (1.0f - kSlack);
-        const float h = d2lo - dw - kSlack * (fabsf(lo.w) + fabsf(sw)) - kTiny32;
+        const float wslack = kSlack * (fabsf(lo.w) + fabsf(sw)) + kTiny32;
+        const float h = d2lo - dw - wslack;
         if (4.0f * d2lo > rho2 && h > 0.0f && h * h > rho2 * d2 * (1.0f + kSlack)) return true;
-        const float S_ = fmaxf(fmaxf(alx * L_[0], alx * U_[0]), fmaxf(ahx * L_[0], ahx * U_[0])) +
-                         fmaxf(fmaxf(aly * L_[1], aly * U_[1]), fmaxf(ahy * L_[1], ahy * U_[1])) +
-                         fmaxf(fmaxf(alz * L_[2], alz * U_[2]), fmaxf(ahz * L_[2], ahz * U_[2]));
-        const float M = fmaxf(fabsf(alx), fabsf(ahx)) * fmaxf(fabsf(L_[0]), fabsf(U_[0])) +
-                        fmaxf(fabsf(aly), fabsf(ahy)) * fmaxf(fabsf(L_[1]), fabsf(U_[1])) +
-                        fmaxf(fabsf(alz), fabsf(ahz)) * fmaxf(fabsf(L_[2]), fabsf(U_[2]));
-        if ((hlow - S_) > kSlack * (d2 + fabsf(lo.w) + fabsf(sw) + M) + kTiny32) return true;
-        return balls_clear(alx, aly, alz, ahx, ahy, ahz, dw + kSlack * (fabsf(lo.w) + fabsf(sw)) + kTiny32);
+        return balls_clear<kSplit>(alx, aly, alz, ahx, ahy, ahz, dw + wslack);
     }
 
-    // Exact form of the prune, per vertex (group op).  Vertex y is beyond the
-    // plane of p iff a . y - |a|^2 + dw > 0, i.e. in doubled coordinates
+    // Exact form of the prune, per vertex.  Vertex y is beyond the plane of
+    // p iff a . y - |a|^2 + dw > 0, i.e. in doubled coordinates
     // |2a - y|^2 < |y|^2 + 4 dw: p lies in the ball about the vertex through
     // the seed (grown by the weight difference).  No point of the box cuts the
     // cell when the box misses every vertex's ball.  The box [2 al, 2 ah] is
     // widened by the cell's largest error radius (each coordinate of y lies
-    // within w of y~), and sum_k (|y~_k| + w)^2 bounds |y|^2.
+    // within w of y~), and sum_k (|y~_k| + w)^2 bounds |y|^2.  The lanes that
+    // test one box (the group, or its half when kSplit) share the vertex
+    // slots.
+    template <bool kSplit>
     __device__ __forceinline__ bool balls_clear(float alx, float aly, float alz, float ahx, float ahy, float ahz,
                                                 float dwmax) const {
+        constexpr int H = kSplit ? G / 2 : G;
+        const int hl = gl & (H - 1);
         const float Ax = fmaf(2.0f, alx, -EM_), Ay = fmaf(2.0f, aly, -EM_), Az = fmaf(2.0f, alz, -EM_);
         const float Bx = fmaf(2.0f, ahx, EM_), By = fmaf(2.0f, ahy, EM_), Bz = fmaf(2.0f, ahz, EM_);
         const float dw4 = 4.0f * dwmax;
         bool cut = false;
 #pragma unroll
-        for (int k = 0; k < kRows; ++k) {
-            if (!own(valid, k)) continue;
-            const int v = gl + G * k;
+        for (int k = 0; k < V / H; ++k) {
+            const int v = hl + H * k;
+            if (!((valid[(H * k) >> 6] >> (((H * k) & 63) + hl)) & 1ull)) continue;
             const float4 p = S->vpos[v];
             const float rx = fabsf(p.x) + p.w, ry = fabsf(p.y) + p.w, rz = fabsf(p.z) + p.w;
             const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) * (1.0f + 4.0f * kSlack);
@@ -581,7 +585,8 @@ struct Group {
             const float d2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
             cut |= !(d2 * (1.0f - kSlack) > r2 + dw4);
         }
-        return !gany(cut);
+        const uint64_t lanes = (H == 64 ? ~0ull : ((1ull << H) - 1ull)) << (gshift + (gl & ~(H - 1)));
+        return (__ballot(cut) & lanes) == 0ull;
     }
 
     // ---------------------------------------------------------- planes
@@ -1002,15 +1007,37 @@ struct Group {
     }
 
     // Wave-wide: does any cell of the wave still need this box?  key: the
-    // smallest bound over the cells that need it.
-    __device__ __forceinline__ bool packet_needs(float4 lo, float4 hi, float &key) const {
+    // smallest bound over the cells that need it; mine: this cell needs it.
+    __device__ __forceinline__ bool packet_needs(float4 lo, float4 hi, float &key, bool &mine) const {
         float k = FLT_MAX;
-        const bool need = live() && !node_prunable(lo, hi, k);
+        const bool need = live() && !node_prunable<false>(lo, hi, k);
+        mine = need;
         const bool any = __ballot(gl == 0 && need) != 0ull;
         float kk = need ? k : FLT_MAX;
         if constexpr (G == 16) kk = fminf(fminf(rl(kk, 0), rl(kk, 16)), fminf(rl(kk, 32), rl(kk, 48)));
         key = uniform(kk);
         return any;
+    }
+
+    // Both children of a node at once (four-cell waves): the low half of each
+    // group tests child 0, the high half child 1.
+    __device__ __forceinline__ void packet_needs2(const Node &nd, bool &n0, bool &n1, float &k0, float &k1, bool &m0,
+                                                  bool &m1) const {
+        static_assert(G == 16, "four cells per wave");
+        constexpr int H = G / 2;
+        const bool second = gl >= H;
+        const float4 lo = second ? nd.lo1 : nd.lo0, hi = second ? nd.hi1 : nd.hi0;
+        float k = FLT_MAX;
+        const bool need = live() && !node_prunable<true>(lo, hi, k);
+        const uint64_t b = __ballot(need);
+        m0 = ((b >> gshift) & 1ull) != 0ull;
+        m1 = ((b >> (gshift + H)) & 1ull) != 0ull;
+        constexpr uint64_t kLead = 0x0001000100010001ull;  // lane 0 of each group
+        n0 = (b & kLead) != 0ull;
+        n1 = (b & (kLead << H)) != 0ull;
+        const float kk = need ? k : FLT_MAX;
+        k0 = uniform(fminf(fminf(rl(kk, 0), rl(kk, 16)), fminf(rl(kk, 32), rl(kk, 48))));
+        k1 = uniform(fminf(fminf(rl(kk, H), rl(kk, 16 + H)), fminf(rl(kk, 32 + H), rl(kk, 48 + H))));
     }
 
     // Best-first packet traversal: the cells of a wave (Morton neighbours)
@@ -1020,21 +1047,29 @@ struct Group {
     // filtering and clipping on its own lanes.  Unpruned children wait in one
     // wave-wide keyed list and the wave always goes on with the smallest key
     // (the order voronoi_bvh.cuh:246-283 keeps with its min-extraction pops),
-    // re-testing a waiting box when it comes out.  A full list sends the
-    // wave's cells to the next tier instead of dropping entries.
+    // re-testing a waiting box when it comes out.  A leaf goes only to the
+    // cells whose own test kept its box (the others could not be cut by its
+    // points).  A full list sends the wave's cells to the next tier instead
+    // of dropping entries.
     __device__ __forceinline__ void traverse() {
         uint32_t ref = uniform(root_ref);
         int sz = 0;
+        bool mine = true;  // this cell needs box ref
         while (true) {
             bool have = false;
             if (is_leaf(ref)) {
-                if (live()) offer_leaf(ref);
+                if (live() && mine) offer_leaf(ref);
             } else {
                 if (lane64 == 0) count(kStatNodes);
                 const Node nd = nodes[ref];
                 float k0, k1;
-                const bool n0 = packet_needs(nd.lo0, nd.hi0, k0);
-                const bool n1 = packet_needs(nd.lo1, nd.hi1, k1);
+                bool m0, m1, n0, n1;
+                if constexpr (G == 16) {
+                    packet_needs2(nd, n0, n1, k0, k1, m0, m1);
+                } else {
+                    n0 = packet_needs(nd.lo0, nd.hi0, k0, m0);
+                    n1 = packet_needs(nd.lo1, nd.hi1, k1, m1);
+                }
                 const uint32_t r0 = uniform(__float_as_uint(nd.hi0.w)), r1 = uniform(__float_as_uint(nd.hi1.w));
                 if (n0 || n1) {
                     const bool near0 = !n1 || (n0 && k0 <= k1);
@@ -1050,6 +1085,7 @@ struct Group {
                     const int im = list_argmin(sz, kmin);
                     if (im < 0 || knear <= kmin) {
                         ref = near0 ? r0 : r1;
+                        mine = near0 ? m0 : m1;
                         have = true;
                     } else {  // a waiting box is nearer: the near child waits too
                         if (sz >= L) {
@@ -1070,7 +1106,7 @@ struct Group {
                 float k;
                 const float4 lo = c ? nd.lo1 : nd.lo0;
                 const float4 hi = c ? nd.hi1 : nd.hi0;
-                if (packet_needs(lo, hi, k)) {
+                if (packet_needs(lo, hi, k, mine)) {
                     ref = uniform(__float_as_uint(hi.w));
                     have = true;
                 }
@@ -1381,7 +1417,7 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     g.flush(prm, cells);
 }
 
-__global__ void __launch_bounds__(kT0Threads) cell_group_kernel(const CellParams prm) { tier0_body<Tier0>(prm); }
+__global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3))) cell_group_kernel(const CellParams prm) { tier0_body<Tier0>(prm); }
 __global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) { tier0_body<Tier0x>(prm); }
 
 // Tier 1: one cell per wave, the queued cells; rows after the tier-0 slab.
