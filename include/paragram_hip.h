@@ -109,14 +109,21 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
 
 /* The library's own kNN pre-clip: row i (Morton-sorted order) of ids holds
  * the sorted indices of point i's k nearest neighbours (itself as padding);
- * n_clouds = 0 for the index of pg_build_index, else the batch's count.  Pass
- * it to pg_compute_cells_ex / pg_compute_cells_batch with guess_stride = k
- * (no offsets): the cells read their rows in the order they are built. */
-int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, void *stream);
+ * n_clouds = 0 for the index of pg_build_index, else the batch's count.
+ * kth_d2 (may be null): float [n], sorted order, the squared distance of the
+ * row's last neighbour (FLT_MAX for a short row).  Pass both to
+ * pg_compute_cells_ex / pg_compute_cells_batch with guess_stride = k (no
+ * offsets): the cells read their rows in the order they are built, and skip
+ * the traversal's candidates nearer than kth_d2 (already offered).
+ * No counterpart in the reference (its kNN output, pwr_bvh.cu:159-215, is
+ * indexed by original id and carries no distances). */
+int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
+                   void *stream);
 
 /* pg_compute_cells with guess_stride (0: guesses as in pg_compute_cells;
- * 1..64: rows of pg_knn_guesses with that stride, guess_offsets unused) and
- * two optional per-cell arrays (original order):
+ * 1..64: rows of pg_knn_guesses with that stride, guess_offsets unused, and
+ * guess_d2 its kth_d2 or null) and two optional per-cell arrays (original
+ * order):
  * init_half_width (input, may be null): a predicted half-width h of each cell
  *   (max-norm distance of its vertices from the seed).  The cell then starts
  *   as the box cut by the cube of half-width h about the seed instead of the
@@ -129,8 +136,8 @@ int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k
  * kNN pre-clip radius of voronoi_ultra.cu:70-85 as the traversal's start. */
 int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                         const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
-                        int32_t seed_window, const float *init_half_width, float *extent, int32_t *rows,
-                        int64_t *row_offsets,
+                        const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
+                        int32_t *rows, int64_t *row_offsets,
                         int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                         int32_t waves, uint32_t *cell_cycles, void *stream);
 
@@ -193,12 +200,13 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
 /* pg_compute_cells over the cells [begin, end) of a batch (any range of the
  * batch, so the chunk plan applies unchanged); the persistent waves take one
  * quad per round from a global counter.  guesses / guess_offsets: optional,
- * as for pg_compute_cells (original ids of the same cloud). */
+ * as for pg_compute_cells (original ids of the same cloud), or rows of
+ * pg_knn_guesses (guess_stride = k, guess_d2 = its kth_d2 or null). */
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
                            int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t guess_stride, int32_t *rows, int64_t *row_offsets, int32_t *counts,
-                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
-                           uint32_t *cell_cycles, void *stream);
+                           int32_t guess_stride, const float *guess_d2, int32_t *rows, int64_t *row_offsets,
+                           int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                           int32_t waves, uint32_t *cell_cycles, void *stream);
 
 /* pg_knn over a batch index: each point's neighbours come from its own cloud
  * (the batch's ids are global); a point alone in its cloud gets its own id as

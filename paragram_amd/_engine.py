@@ -61,19 +61,19 @@ class HipLibrary:
             "pg_fill_u32": ([vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
             "pg_knn": ([i64, vp, i32, vp, vp, vp], ctypes.c_int),
             "pg_knn_batch": ([i64, vp, i32, i32, vp, vp, vp], ctypes.c_int),
-            "pg_knn_guesses": ([i64, vp, i32, i32, vp, vp], ctypes.c_int),
+            "pg_knn_guesses": ([i64, vp, i32, i32, vp, vp, vp], ctypes.c_int),
             "pg_mlp_param_count": ([], ctypes.c_int),
             "pg_mlp_features": ([i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_mlp_forward": ([i64, vp, vp, vp, vp], ctypes.c_int),
             "pg_mlp_train_scratch_elems": ([i64, u64p], ctypes.c_int),
             "pg_mlp_train_step": ([i64, vp, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp, ctypes.c_float, i32, vp],
                                   ctypes.c_int),
-            "pg_compute_cells_ex": ([i64, vp, i64, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+            "pg_compute_cells_ex": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
                                      ctypes.c_uint64, i32, vp, vp], ctypes.c_int),
             "pg_batch_workspace_bytes": ([i64, i32, u64p], ctypes.c_int),
             "pg_build_batch_index": ([i64, i32, vp, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
             "pg_batch_quads": ([i32, vp, i64, i64, vp], ctypes.c_int),
-            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp,
+            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp,
                                         ctypes.c_uint64, i32, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
@@ -172,6 +172,7 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         guesses = initial_guesses
         goffs = initial_guesses_offsets
         guess_stride = 0
+        guess_d2 = None
         seed_window = 0
         if guesses is None and knn > 0:
             # The reference pre-clips each cell by its knn nearest neighbours
@@ -180,7 +181,9 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             # fastest on the 1M uniform and 2M power clouds.
             guess_stride = KNN_PRECLIP if preclip is None else int(preclip)
             guesses = torch.empty(n * guess_stride, **i32)
-            lib.call("pg_knn_guesses", n, _p(ws), 0 if clouds is None else n_clouds, guess_stride, _p(guesses), sp)
+            guess_d2 = torch.empty(n, dtype=torch.float32, device=dev)
+            lib.call("pg_knn_guesses", n, _p(ws), 0 if clouds is None else n_clouds, guess_stride, _p(guesses),
+                     _p(guess_d2), sp)
             goffs = None
         if initial_guesses is not None and initial_guesses_offsets is not None:
             # the kernel indexes guesses through the offsets: reject malformed CSR
@@ -208,11 +211,11 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             cyc = _p(instrument.cycles if instrument is not None else None)
             if clouds is None:
                 lib.call("pg_compute_cells_ex", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, guess_stride,
-                         seed_window, _p(init_h), _p(extent), _p(rows), _p(row_offsets), _p(counts), _p(status),
+                         _p(guess_d2), seed_window, _p(init_h), _p(extent), _p(rows), _p(row_offsets), _p(counts), _p(status),
                          _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
             else:
                 lib.call("pg_compute_cells_batch", n, _p(ws), n_clouds, c_off, begin, end, _p(guesses), _p(goffs),
-                         guess_stride, _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch),
+                         guess_stride, _p(guess_d2), _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch),
                          scratch_bytes, waves, cyc, sp)
             if ev is not None:
                 ev[1].record(stream)

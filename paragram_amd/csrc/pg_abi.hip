@@ -168,13 +168,14 @@ int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, 
     return check_hip(pg::knn(n, workspace, k, ids, d2, as_stream(stream), n_clouds), "knn_batch");
 }
 
-int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, void *stream) {
+int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
+                   void *stream) {
     if (int r = check_n(n)) return r;
     if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
     if (n_clouds < 0 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [0, 4096]");
     if (n == 0) return 0;
     if (!workspace || !ids) return fail(1, "workspace and ids must be non-null");
-    return check_hip(pg::knn(n, workspace, k, ids, nullptr, as_stream(stream), n_clouds, true), "knn_guesses");
+    return check_hip(pg::knn(n, workspace, k, ids, kth_d2, as_stream(stream), n_clouds, true), "knn_guesses");
 }
 
 int pg_default_waves(int32_t *waves) {
@@ -209,15 +210,15 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      int64_t *row_offsets,
                      int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                      int32_t waves, uint32_t *cell_cycles, void *stream) {
-    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, 0, seed_window,
-                               nullptr, nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes,
+    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, 0, nullptr,
+                               seed_window, nullptr, nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes,
                                waves, cell_cycles, stream);
 }
 
 int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                         const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
-                        int32_t seed_window, const float *init_half_width, float *extent, int32_t *rows,
-                        int64_t *row_offsets,
+                        const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
+                        int32_t *rows, int64_t *row_offsets,
                         int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
                         int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
@@ -237,6 +238,7 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
         return fail(1, "cell scratch too small");
     pg::CellArgs a;
     a.guess_stride = guess_stride;
+    a.guess_d2 = guess_stride > 0 ? guess_d2 : nullptr;
     a.n = n;
     a.index_ws = workspace;
     a.begin = begin;
@@ -265,9 +267,9 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
 
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
                            int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t guess_stride, int32_t *rows, int64_t *row_offsets, int32_t *counts,
-                           int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
-                           uint32_t *cell_cycles, void *stream) {
+                           int32_t guess_stride, const float *guess_d2, int32_t *rows, int64_t *row_offsets,
+                           int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                           int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
@@ -299,6 +301,7 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
         return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
     if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
     a.guess_stride = guess_stride;
+    a.guess_d2 = guess_stride > 0 ? guess_d2 : nullptr;
     a.guesses = guesses;
     a.guess_offsets = guess_offsets;
     a.guesses_sorted = 0;

@@ -51,6 +51,7 @@ struct CellParams {
     const int32_t *guess_offsets;
     int guesses_sorted;
     int guess_stride;             // > 0: guesses are Morton-ordered rows of this many sorted ids
+    const float *guess_d2;        // with guess_stride (may be null): squared distance of each row's last id
     int seed_window;              // Morton neighbours offered first (0: none)
     int32_t *rows;                // tier-0 slab (row_stride per cell) then tier-1 rows
     int row_stride;
@@ -134,6 +135,7 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.guess_offsets = a.guess_offsets;
     prm.guesses_sorted = a.guesses_sorted;
     prm.guess_stride = a.guess_stride;
+    prm.guess_d2 = a.guess_d2;
     prm.seed_window = a.seed_window;
     prm.rows = a.rows;
     prm.row_stride = a.row_stride;

@@ -331,6 +331,9 @@ struct Group {
     bool dirty;   // the cell changed since the AABB was computed
     uint64_t bloom;
     bool in_guess;
+    // After the pre-clip: every point nearer the seed than this (squared,
+    // lowered by a margin) is one of the guesses already offered.
+    float guess_r2;
 
     // diagnostics counters live in LDS (registers are the scarce resource)
     __device__ __forceinline__ void count(int slot, uint32_t v = 1u) const { atomicAdd(&S->ctr[slot], v); }
@@ -840,6 +843,7 @@ struct Group {
             return false;
         }
         const float a2 = c.ax * c.ax + c.ay * c.ay + c.az * c.az;
+        if (a2 < guess_r2) return false;  // offered by the pre-clip; the cell has only shrunk since
         c.H = a2 - dw;
         c.pH = a2 + fabsf(dw);
         c.pa1 = fabsf(c.ax) + fabsf(c.ay) + fabsf(c.az);
@@ -1214,6 +1218,7 @@ struct Group {
         const uint32_t orig = perm[cell];
         init_box((!kExact && prm.init_h) ? 2.0f * prm.init_h[orig] : 0.0f);
         in_guess = true;
+        guess_r2 = 0.0f;
         if (!prm.guesses && prm.seed_window > 0) {
             // Morton neighbours first: the cheap analogue of clipping by the
             // k nearest neighbours (voronoi_ultra.cu:70-85); order does not
@@ -1241,6 +1246,11 @@ struct Group {
                 }
                 offer_batch(has, j);
             }
+            // The row holds the guess_stride nearest points (kNN order, ties
+            // and all); a point off the row has squared distance >= the
+            // row's last, up to the few ulps of the two FP32 evaluations of
+            // |p - s|^2, which the 1e-5 relative margin covers.
+            if (strided && prm.guess_d2) guess_r2 = prm.guess_d2[cell] * (1.0f - 1e-5f);
         }
         in_guess = false;
     }

@@ -32,6 +32,7 @@ struct CellArgs {
     const int32_t *guess_offsets; // indexed by original id (null with guesses == null)
     int guesses_sorted;           // guesses already hold sorted indices (kNN output)
     int guess_stride = 0;         // > 0: Morton-ordered rows of this many sorted ids (offsets unused)
+    const float *guess_d2 = nullptr;  // with guess_stride: squared distance of each row's last id (sorted order)
     int seed_window;              // Morton neighbours offered before the traversal
     int32_t *rows;                // (end - begin) x row_stride, big_cap x kTier1Row, huge_cap x kTier2Row
     int row_stride;

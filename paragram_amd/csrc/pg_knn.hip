@@ -106,14 +106,20 @@ __global__ void __launch_bounds__(kKnnThreads)
     }
     // short lists repeat the last id found; with none found the reference
     // writes 0, a batch writes the point itself (0 may be another cloud's).
-    // sorted_rows: row i (sorted order) of sorted ids, self as padding.
+    // sorted_rows: row i (sorted order) of sorted ids, self as padding, and
+    // (out_d2 non-null) out_d2[i] = the squared distance of the row's last
+    // neighbour, FLT_MAX when the row is short (it then holds every other
+    // point the search could reach).
     if (sorted_rows) {
         const int64_t o = i * k_out;
         int32_t last = (int32_t)i;
+        float dk = FLT_MAX;
         for (int t = 0; t < k_out; ++t) {
             if (id[t] >= 0) last = id[t];
             out_ids[o + t] = last;
+            dk = id[t] >= 0 ? d[t] : FLT_MAX;
         }
+        if (out_d2) out_d2[i] = dk;
         return;
     }
     const int64_t o = (int64_t)perm[i] * k_out;

@@ -56,6 +56,7 @@ class GraphedDiagram:
         self.scan_tmp = torch.empty(max(lib.u64("pg_scan_tmp_bytes", n + 1), 4), dtype=torch.uint8, device=dev)
         self.stride = _engine.KNN_PRECLIP if knn > 0 else 0
         self.guesses = torch.empty(max(n * self.stride, 1), **i32)
+        self.guess_d2 = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
         self.leaf_size = leaf_size
         self._pipeline()  # once eagerly (first-use work outside the capture)
         torch.cuda.synchronize(dev)
@@ -70,11 +71,11 @@ class GraphedDiagram:
         lib.call("pg_fill_u32", _p(self.stats), 2 * _engine.NUM_STATS, 0, sp)
         lib.call("pg_build_index", n, _p(self.points), _p(self.weights), int(self.leaf_size), _p(self.ws),
                  self.ws_bytes, sp)
-        g = None
+        g = gd = None
         if self.stride:
-            lib.call("pg_knn_guesses", n, _p(self.ws), 0, self.stride, _p(self.guesses), sp)
-            g = self.guesses
-        lib.call("pg_compute_cells_ex", n, _p(self.ws), 0, n, _p(g), None, 0, self.stride, 0, None, None,
+            lib.call("pg_knn_guesses", n, _p(self.ws), 0, self.stride, _p(self.guesses), _p(self.guess_d2), sp)
+            g, gd = self.guesses, self.guess_d2
+        lib.call("pg_compute_cells_ex", n, _p(self.ws), 0, n, _p(g), None, 0, self.stride, _p(gd), 0, None, None,
                  _p(self.rows), _p(self.row_offsets), _p(self.counts), _p(self.status), _p(self.stats),
                  _p(self.scratch), self.scratch_bytes, self.waves, None, sp)
         lib.call("pg_scan_counts", n, _p(self.counts), _p(self.offsets), _p(self.scan_tmp), sp)
