@@ -20,6 +20,8 @@
 //   plx[P]   float2  permanent of H and the owner (sorted point id, or a
 //                    negative box-plane code)
 //   tmp[V]   uint32  clip and emission scratch
+//   pm[P/2]  uint32  (tier 0) per plane, a 16-bit mask of the lanes whose
+//                    removed triangle holds the plane; zero between clips
 // Vertices live in a pool of V slots with a group-uniform valid mask (slot
 // v = lane + G k belongs to lane v mod G), so a clip frees and refills slots
 // without moving data.
@@ -40,6 +42,7 @@ struct alignas(16) GroupLDS {
     float2 plx[P];
     uint32_t vtri[V];
     uint32_t tmp[V];
+    uint32_t pm[P <= 64 ? (P + 1) / 2 : 1];
     uint32_t ctr[12];  // diagnostics, in the order of the statistics slots
 };
 
@@ -735,11 +738,37 @@ struct Group {
             tr = has_tri ? S->tmp[gl] : 0u;
             const uint32_t t0 = tri_get<kB>(tr, 0), t1 = tri_get<kB>(tr, 1), t2 = tri_get<kB>(tr, 2);
             bool tw0 = false, tw1 = false, tw2 = false;
-            for (int u = 0; u < R; ++u) {  // broadcast reads of the removed triangles
-                const uint32_t o = S->tmp[u];
-                tw0 |= tri_has_edge<kB>(o, t1, t0);
-                tw1 |= tri_has_edge<kB>(o, t2, t1);
-                tw2 |= tri_has_edge<kB>(o, t0, t2);
+            if constexpr (G == 16 && P <= 64) {
+                // The dual of a cell is a triangulated sphere: a pair of
+                // planes lies in at most two triangles, which then share it
+                // as an edge.  So the twin of edge (x, y) is removed iff
+                // another removed triangle holds both x and y: per plane, OR
+                // in the lanes whose removed triangle holds it (LDS
+                // operations of one wave complete in order), intersect, and
+                // clear the words again.
+                const uint32_t bit = 1u << gl;
+                if (has_tri) {
+                    atomicOr(&S->pm[t0 >> 1], bit << ((t0 & 1u) << 4));
+                    atomicOr(&S->pm[t1 >> 1], bit << ((t1 & 1u) << 4));
+                    atomicOr(&S->pm[t2 >> 1], bit << ((t2 & 1u) << 4));
+                    const uint32_t m0 = S->pm[t0 >> 1] >> ((t0 & 1u) << 4);
+                    const uint32_t m1 = S->pm[t1 >> 1] >> ((t1 & 1u) << 4);
+                    const uint32_t m2 = S->pm[t2 >> 1] >> ((t2 & 1u) << 4);
+                    const uint32_t others = 0xFFFFu & ~bit;
+                    tw0 = (m0 & m1 & others) != 0u;
+                    tw1 = (m1 & m2 & others) != 0u;
+                    tw2 = (m2 & m0 & others) != 0u;
+                    S->pm[t0 >> 1] = 0u;
+                    S->pm[t1 >> 1] = 0u;
+                    S->pm[t2 >> 1] = 0u;
+                }
+            } else {
+                for (int u = 0; u < R; ++u) {  // broadcast reads of the removed triangles
+                    const uint32_t o = S->tmp[u];
+                    tw0 |= tri_has_edge<kB>(o, t1, t0);
+                    tw1 |= tri_has_edge<kB>(o, t2, t1);
+                    tw2 |= tri_has_edge<kB>(o, t0, t2);
+                }
             }
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
@@ -1271,6 +1300,7 @@ struct Group {
         nodes = prm.nodes;
         set_cloud(prm.hdr);
         if (gl < 12) S->ctr[gl] = 0u;
+        for (int w = gl; w < (int)(sizeof(S->pm) / 4); w += G) S->pm[w] = 0u;
     }
 
     // The cloud the next cells belong to (its clipping box and subtree root).
