@@ -72,8 +72,10 @@ int pg_build_index(int64_t n, const float *points, const float *weights, int32_t
  * pg_build_index: ids[i*k + t] (original ids, ascending squared distance,
  * the point itself excluded, a short list padded with its last id or 0) and,
  * if d2 is non-null, the squared distances (FLT_MAX for padding).
+ * The index workspace's sort scratch holds the queries whose search needs a
+ * deep stack (the index itself is not changed).
  * Replaces cuBQL_knn_kernel (pwr_bvh.cu:159-215). */
-int pg_knn(int64_t n, const void *workspace, int32_t k, int32_t *ids, float *d2, void *stream);
+int pg_knn(int64_t n, void *workspace, int32_t k, int32_t *ids, float *d2, void *stream);
 
 /* Persistent waves used by pg_compute_cells on the current device. */
 int pg_default_waves(int32_t *waves);
@@ -117,7 +119,7 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
  * the traversal's candidates nearer than kth_d2 (already offered).
  * No counterpart in the reference (its kNN output, pwr_bvh.cu:159-215, is
  * indexed by original id and carries no distances). */
-int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
+int pg_knn_guesses(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
                    void *stream);
 
 /* pg_compute_cells with guess_stride (0: guesses as in pg_compute_cells;
@@ -211,7 +213,7 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
 /* pg_knn over a batch index: each point's neighbours come from its own cloud
  * (the batch's ids are global); a point alone in its cloud gets its own id as
  * padding (the cell kernels skip it). */
-int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
+int pg_knn_batch(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
                  void *stream);
 
 /* Bytes of temporary storage for pg_scan_counts over n values. */

@@ -114,7 +114,7 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
     return 0;
 }
 
-int pg_knn(int64_t n, const void *workspace, int32_t k, int32_t *ids, float *d2, void *stream) {
+int pg_knn(int64_t n, void *workspace, int32_t k, int32_t *ids, float *d2, void *stream) {
     if (int r = check_n(n)) return r;
     if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
     if (n == 0) return 0;
@@ -158,7 +158,7 @@ int pg_mlp_train_step(int64_t n, const float *features, const float *targets, co
                      "mlp_train_step");
 }
 
-int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
+int pg_knn_batch(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
                  void *stream) {
     if (int r = check_n(n)) return r;
     if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");
@@ -168,7 +168,7 @@ int pg_knn_batch(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, 
     return check_hip(pg::knn(n, workspace, k, ids, d2, as_stream(stream), n_clouds), "knn_batch");
 }
 
-int pg_knn_guesses(int64_t n, const void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
+int pg_knn_guesses(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *kth_d2,
                    void *stream) {
     if (int r = check_n(n)) return r;
     if (k < 1 || k > 16) return fail(1, "k must be in [1, 16]");

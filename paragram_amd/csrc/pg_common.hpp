@@ -92,6 +92,7 @@ struct IndexHeader {
     uint32_t has_weights;
     uint32_t leaf_size;
     uint32_t has_duplicates;      // some points coincide in position and weight
+    uint32_t knn_spill;           // kNN queries left to the deep-stack pass (see knn)
 };
 
 // Offsets of the arrays inside the index workspace.
@@ -103,7 +104,7 @@ struct IndexLayout {
     uint64_t nodes;      // Node [max(n-1,1)]
     uint64_t keys;       // uint32 [n]  sort keys (ping)
     uint64_t keys_alt;   // uint32 [n]  (pong)
-    uint64_t vals_alt;   // uint32 [n]
+    uint64_t vals_alt;   // uint32 [n]  (after the build: the kNN pass's spilled queries)
     uint64_t hist;       // uint32 [256 * sort_blocks + 256]
     uint64_t children;   // uint32 [2*(n-1)]  Karras child codes
     uint64_t parents;    // uint32 [2n-1]     parent of internal (0..n-2) and leaf (n-1..2n-2)

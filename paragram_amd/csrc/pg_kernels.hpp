@@ -15,7 +15,7 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
                        hipStream_t stream);
 hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
                              const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream);
-hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
+hipError_t knn(int64_t n, void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
                int n_clouds = 0, bool sorted_rows = false);
 hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
                         float *logd8, hipStream_t stream);

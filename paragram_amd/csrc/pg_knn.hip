@@ -9,6 +9,9 @@
 // re-tested against the current K-th distance when it comes out.  A Karras
 // hierarchy over 32-bit keys and < 2^26 points is less than 64 levels deep,
 // and every level pushes at most one entry, so 64 entries never overflow.
+// The first pass keeps 32 (half the LDS, so more waves per CU); the rare
+// query that needs more is queued in the index workspace and redone by a
+// second pass with 64.
 #include "pg_common.hpp"
 #include "pg_kernels.hpp"
 
@@ -17,7 +20,8 @@
 namespace pg {
 
 constexpr int kKnnThreads = 128;
-constexpr int kKnnStack = 64;
+constexpr int kKnnStack = 64;      // deep pass: never overflows (see above)
+constexpr int kKnnStackFast = 32;  // first pass: 16 KB of LDS per block instead of 32 KB
 
 __device__ __forceinline__ float box_dist2(float4 lo, float4 hi, float qx, float qy, float qz) {
     const float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
@@ -26,15 +30,26 @@ __device__ __forceinline__ float box_dist2(float4 lo, float4 hi, float qx, float
     return dx * dx + dy * dy + dz * dz;
 }
 
-template <int K>
+// S: stack entries.  The first pass (kDeep false) runs every query with a
+// 32-entry stack; a query whose path needs more appends itself to `spill`
+// and the deep pass (kDeep, 64 entries) redoes it, so no query's result
+// depends on the stack size.
+template <int K, int S, bool kDeep>
 __global__ void __launch_bounds__(kKnnThreads)
-    knn_kernel(int64_t n, const IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
+    knn_kernel(int64_t n, IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
                const uint32_t *__restrict__ perm, const Node *__restrict__ nodes, int k_out,
                int32_t *__restrict__ out_ids, float *__restrict__ out_d2, const IndexHeader *__restrict__ clouds,
-               const int64_t *__restrict__ cloud_cells, int n_clouds, int sorted_rows) {
-    __shared__ uint32_t stack[kKnnStack][kKnnThreads];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+               const int64_t *__restrict__ cloud_cells, int n_clouds, int sorted_rows, uint32_t *__restrict__ spill) {
+    __shared__ uint32_t stack[S][kKnnThreads];
+    int64_t i;
+    if constexpr (kDeep) {
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= (int64_t)hdr->knn_spill) return;
+        i = spill[t];
+    } else {
+        i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+    }
     const float4 q = pts[i];
     float d[K];
     int32_t id[K];
@@ -43,6 +58,25 @@ __global__ void __launch_bounds__(kKnnThreads)
     // a node is pruned when its box is farther than the K-th distance (with
     // a relative margin for the rounding of the two squared distances)
     auto far = [&](float bd) { return bd * (1.0f - 1e-6f) > d[K - 1]; };
+    auto offer = [&](const float4 &p, uint32_t j) {
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float dd = dx * dx + dy * dy + dz * dz;
+        if (dd < d[K - 1]) {  // sorted insertion, ties keep the earlier point
+            d[K - 1] = dd;
+            id[K - 1] = (int32_t)j;
+#pragma unroll
+            for (int u = K - 1; u > 0; --u) {
+                if (d[u] < d[u - 1]) {
+                    const float td = d[u];
+                    d[u] = d[u - 1];
+                    d[u - 1] = td;
+                    const int32_t ti = id[u];
+                    id[u] = id[u - 1];
+                    id[u - 1] = ti;
+                }
+            }
+        }
+    };
     // a batch searches the query's own cloud (its subtree)
     uint32_t ref = hdr->root_ref;
     if (clouds != nullptr) {
@@ -55,29 +89,19 @@ __global__ void __launch_bounds__(kKnnThreads)
         ref = clouds[lo].root_ref;
     }
     int sp = 0;
+    bool spilled = false;
     while (true) {
         if (is_leaf(ref)) {
             const uint32_t f = leaf_first(ref), c = leaf_count(ref);
-            for (uint32_t t = 0; t < c; ++t) {
-                const uint32_t j = f + t;
-                if ((int64_t)j == i) continue;
-                const float4 p = pts[j];
-                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                const float dd = dx * dx + dy * dy + dz * dz;
-                if (dd < d[K - 1]) {  // sorted insertion, ties keep the earlier point
-                    d[K - 1] = dd;
-                    id[K - 1] = (int32_t)j;
+            // four loads in flight, then four offers (leaf order kept)
+            for (uint32_t t = 0; t < c; t += 4) {
+                float4 p[4];
 #pragma unroll
-                    for (int u = K - 1; u > 0; --u) {
-                        if (d[u] < d[u - 1]) {
-                            const float td = d[u];
-                            d[u] = d[u - 1];
-                            d[u - 1] = td;
-                            const int32_t ti = id[u];
-                            id[u] = id[u - 1];
-                            id[u - 1] = ti;
-                        }
-                    }
+                for (int u = 0; u < 4; ++u) p[u] = pts[f + min(t + (uint32_t)u, c - 1u)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t j = f + t + (uint32_t)u;
+                    if (t + (uint32_t)u < c && (int64_t)j != i) offer(p[u], j);
                 }
             }
         } else {
@@ -86,7 +110,13 @@ __global__ void __launch_bounds__(kKnnThreads)
             const float b1 = box_dist2(nd.lo1, nd.hi1, q.x, q.y, q.z);
             const bool near0 = b0 <= b1;
             const float bn = near0 ? b0 : b1, bf = near0 ? b1 : b0;
-            if (!far(bf) && sp < kKnnStack) stack[sp++][threadIdx.x] = (ref << 1) | (near0 ? 1u : 0u);
+            if (!far(bf)) {
+                if (sp == S) {
+                    spilled = true;
+                    break;
+                }
+                stack[sp++][threadIdx.x] = (ref << 1) | (near0 ? 1u : 0u);
+            }
             if (!far(bn)) {
                 ref = __float_as_uint(near0 ? nd.hi0.w : nd.hi1.w);
                 continue;
@@ -103,6 +133,10 @@ __global__ void __launch_bounds__(kKnnThreads)
             }
         }
         if (!have) break;
+    }
+    if (!kDeep && spilled) {
+        spill[atomicAdd(&hdr->knn_spill, 1u)] = (uint32_t)i;
+        return;
     }
     // short lists repeat the last id found; with none found the reference
     // writes 0, a batch writes the point itself (0 may be another cloud's).
@@ -131,11 +165,27 @@ __global__ void __launch_bounds__(kKnnThreads)
     }
 }
 
-hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds,
+__global__ void knn_reset_kernel(IndexHeader *hdr) { hdr->knn_spill = 0u; }
+
+template <int K>
+static void launch_knn(int64_t n, IndexHeader *hdr, const float4 *pts, const uint32_t *perm, const Node *nodes, int k,
+                       int32_t *ids, float *d2, const IndexHeader *clouds, const int64_t *cloud_cells, int n_clouds,
+                       int sorted, uint32_t *spill, hipStream_t stream) {
+    const unsigned grid = (unsigned)((n + kKnnThreads - 1) / kKnnThreads);
+    hipLaunchKernelGGL(knn_reset_kernel, dim3(1), dim3(1), 0, stream, hdr);
+    hipLaunchKernelGGL((knn_kernel<K, kKnnStackFast, false>), dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts,
+                       perm, nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted, spill);
+    // the deep pass reads the spill count on the device; its grid covers
+    // every query in case all of them spilled (blocks past the count exit)
+    hipLaunchKernelGGL((knn_kernel<K, kKnnStack, true>), dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm,
+                       nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted, spill);
+}
+
+hipError_t knn(int64_t n, void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds,
                bool sorted_rows) {
     if (n <= 0) return hipSuccess;
     const IndexLayout L = make_index_layout(n);
-    const char *ws = (const char *)index_ws;
+    char *ws = (char *)index_ws;
     const IndexHeader *clouds = nullptr;
     const int64_t *cloud_cells = nullptr;
     if (n_clouds > 0) {
@@ -143,17 +193,17 @@ hipError_t knn(int64_t n, const void *index_ws, int k, int32_t *ids, float *d2, 
         clouds = (const IndexHeader *)(ws + B.clouds);
         cloud_cells = (const int64_t *)(ws + B.offsets);
     }
-    const IndexHeader *hdr = (const IndexHeader *)(ws + L.header);
+    IndexHeader *hdr = (IndexHeader *)(ws + L.header);
     const float4 *pts = (const float4 *)(ws + L.points);
     const uint32_t *perm = (const uint32_t *)(ws + L.perm);
     const Node *nodes = (const Node *)(ws + L.nodes);
-    const unsigned grid = (unsigned)((n + kKnnThreads - 1) / kKnnThreads);
+    uint32_t *spill = (uint32_t *)(ws + L.vals_alt);
     if (k <= 8)
-        hipLaunchKernelGGL(knn_kernel<8>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k, ids,
-                           d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0);
+        launch_knn<8>(n, hdr, pts, perm, nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0, spill,
+                      stream);
     else
-        hipLaunchKernelGGL(knn_kernel<16>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k,
-                           ids, d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0);
+        launch_knn<16>(n, hdr, pts, perm, nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted_rows ? 1 : 0, spill,
+                       stream);
     return hipGetLastError();
 }
 

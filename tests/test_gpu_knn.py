@@ -77,3 +77,52 @@ def test_knn_duplicates_are_neighbours(gpu):
     for i in range(50):
         assert d2[i, 0] == 0.0 and ids[i, 0] == 500 + i
         assert d2[500 + i, 0] == 0.0 and ids[500 + i, 0] == i
+
+
+def chain_cloud():
+    """A hierarchy deeper than the first pass's 32-entry stack: 4096
+    coincident points at the origin (split by index below the 30 Morton bit
+    levels) and one point on each Morton bit level (x, y or z = 2^-k), so the
+    path to the origin cluster is about 37 levels, and a first descent (every
+    far child still within the unset k-th distance) pushes at each."""
+    pts = [np.zeros((4096, 3))]
+    for k in range(1, 11):
+        for axis in range(3):
+            p = np.zeros(3)
+            p[axis] = 2.0 ** -k
+            pts.append(p[None])
+    pts.append(np.ones((1, 3)))
+    return np.concatenate(pts).astype(np.float32)
+
+
+def test_knn_deep_hierarchy_spills_to_the_deep_pass(gpu):
+    import torch
+
+    from paragram_amd import _backend, _engine
+
+    print("knn on a 37-level hierarchy (first-pass stack overflow -> deep pass)", flush=True)
+    pts = chain_cloud()
+    n, k = pts.shape[0], 16
+    lib = _backend._C.lib
+    P = torch.from_numpy(pts).to(gpu)
+    ws = torch.empty(lib.u64("pg_index_workspace_bytes", n), dtype=torch.uint8, device=gpu)
+    ids = torch.empty((n, k), dtype=torch.int32, device=gpu)
+    d2 = torch.empty((n, k), dtype=torch.float32, device=gpu)
+    import ctypes
+
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib.call("pg_build_index", n, _engine._p(P), None, -1, _engine._p(ws), ws.numel(), sp)
+    lib.call("pg_knn", n, _engine._p(ws), k, _engine._p(ids), _engine._p(d2), sp)
+    torch.cuda.synchronize()
+    # IndexHeader: 12 floats, then root_ref, n, has_weights, leaf_size,
+    # has_duplicates, knn_spill (uint32 each): knn_spill at byte 68
+    spilled = int(ws[68:72].cpu().numpy().view(np.uint32)[0])
+    print(f"queries redone by the deep pass: {spilled}", flush=True)
+    assert spilled > 0, "the dataset did not reach the deep pass"
+    ids, d2 = ids.cpu().numpy(), d2.cpu().numpy()
+    _, bd2 = brute(pts, k)
+    assert np.allclose(d2, bd2, rtol=2e-6, atol=0), "k nearest squared distances differ"
+    p = pts.astype(np.float64)
+    got = ((p[ids] - p[:, None, :]) ** 2).sum(-1)
+    assert np.allclose(got, bd2, rtol=2e-6, atol=0), "returned ids are not at the returned distances"
+    assert (ids != np.arange(n)[:, None]).all(), "a point is its own neighbour"
