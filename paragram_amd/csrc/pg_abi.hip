@@ -369,8 +369,9 @@ int pg_check_symmetry(int64_t n, const void *workspace, const int32_t *counts, c
     if (!workspace || !counts || !src || !offsets || !stats) return fail(1, "invalid argument");
     if (!src_offsets && src_stride <= 0) return fail(1, "src_stride must be positive without src_offsets");
     const pg::IndexLayout L = pg::make_index_layout(n);
+    const uint32_t *perm = (const uint32_t *)((const char *)workspace + L.perm);
     const uint32_t *inv = (const uint32_t *)((const char *)workspace + L.inv);
-    return check_hip(pg::check_symmetry(n, inv, counts, src, src_offsets, src_stride, offsets,
+    return check_hip(pg::check_symmetry(n, perm, inv, counts, src, src_offsets, src_stride, offsets,
                                         (unsigned long long *)stats, as_stream(stream)),
                      "check_symmetry");
 }

@@ -151,15 +151,18 @@ __device__ __forceinline__ bool row_contains(const int32_t *row, int c, int32_t 
 
 // Counts directed edges u->v whose reverse v->u is missing, and entries that
 // point at empty cells (adjacency.cuh:106-107), plus copies the edge total.
-__global__ void check_symmetry_kernel(int64_t n, const uint32_t *__restrict__ inv, const int32_t *__restrict__ counts,
-                                      const int32_t *__restrict__ src, const int64_t *__restrict__ src_offsets,
-                                      int64_t src_stride, const int32_t *__restrict__ offsets,
-                                      unsigned long long *__restrict__ stats) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One thread per cell in sorted (Morton) order: the rows a wave reads, its
+// own and its neighbours', are Morton neighbours in the row buffer.
+__global__ void check_symmetry_kernel(int64_t n, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ inv,
+                                      const int32_t *__restrict__ counts, const int32_t *__restrict__ src,
+                                      const int64_t *__restrict__ src_offsets, int64_t src_stride,
+                                      const int32_t *__restrict__ offsets, unsigned long long *__restrict__ stats) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t asym = 0, dropped = 0;
-    if (u < n) {
+    if (t < n) {
+        const uint32_t u = perm[t];
         const int c = counts[u];
-        const int32_t *ru = src + src_off(src_offsets, src_stride, inv[u]);
+        const int32_t *ru = src + src_off(src_offsets, src_stride, (uint32_t)t);
         for (int k = 0; k < c; ++k) {
             const int32_t v = ru[k];
             const int cv = counts[v];
@@ -179,7 +182,7 @@ __global__ void check_symmetry_kernel(int64_t n, const uint32_t *__restrict__ in
         if (asym) atomicAdd(&stats[kStatAsymmetric], (unsigned long long)asym);
         if (dropped) atomicAdd(&stats[kStatDropped], (unsigned long long)dropped);
     }
-    if (u == 0) stats[15] = (unsigned long long)(uint32_t)offsets[n];
+    if (t == 0) stats[15] = (unsigned long long)(uint32_t)offsets[n];
 }
 
 // Union symmetrisation on the emitted CSR (adjacency.cuh:78-159 semantics):
@@ -274,11 +277,11 @@ hipError_t emit_rows(int64_t n, const uint32_t *inv, const int32_t *counts, cons
     return hipGetLastError();
 }
 
-hipError_t check_symmetry(int64_t n, const uint32_t *inv, const int32_t *counts, const int32_t *src,
-                          const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets,
+hipError_t check_symmetry(int64_t n, const uint32_t *perm, const uint32_t *inv, const int32_t *counts,
+                          const int32_t *src, const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets,
                           unsigned long long *stats, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(check_symmetry_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, n, inv, counts, src,
+    hipLaunchKernelGGL(check_symmetry_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, n, perm, inv, counts, src,
                        src_offsets, src_stride, offsets, stats);
     return hipGetLastError();
 }

@@ -63,7 +63,7 @@ hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int
 hipError_t emit_rows(int64_t n, const uint32_t *inv, const int32_t *counts, const int32_t *offsets,
                      const int32_t *src, const int64_t *src_offsets, int64_t src_stride, int32_t *adjacency,
                      hipStream_t stream);
-hipError_t check_symmetry(int64_t n, const uint32_t *inv, const int32_t *counts, const int32_t *src,
+hipError_t check_symmetry(int64_t n, const uint32_t *perm, const uint32_t *inv, const int32_t *counts, const int32_t *src,
                           const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets,
                           unsigned long long *stats, hipStream_t stream);
 hipError_t pack_rows(int64_t begin, int64_t end, const uint32_t *perm, const int32_t *counts,
