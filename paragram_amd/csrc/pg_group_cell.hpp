@@ -585,9 +585,12 @@ struct Group {
             const float4 p = S->vpos[v];
             const float rx = fabsf(p.x) + p.w, ry = fabsf(p.y) + p.w, rz = fabsf(p.z) + p.w;
             const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) * (1.0f + 4.0f * kSlack);
-            const float tx = fmaxf(fmaxf(Ax - p.x, p.x - Bx), 0.0f);
-            const float ty = fmaxf(fmaxf(Ay - p.y, p.y - By), 0.0f);
-            const float tz = fmaxf(fmaxf(Az - p.z, p.z - Bz), 0.0f);
+            // distance to the box per axis: y minus y clamped to [A, B]
+            // (A <= B; the same difference as A - y or y - B, so the same
+            // rounding)
+            const float tx = p.x - __builtin_amdgcn_fmed3f(p.x, Ax, Bx);
+            const float ty = p.y - __builtin_amdgcn_fmed3f(p.y, Ay, By);
+            const float tz = p.z - __builtin_amdgcn_fmed3f(p.z, Az, Bz);
             const float d2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
             cut |= !(d2 * (1.0f - kSlack) > r2 + dw4);
         }
