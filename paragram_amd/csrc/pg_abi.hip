@@ -256,7 +256,7 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     a.scratch = scratch;
     a.scratch_bytes = scratch_bytes;
     a.waves = w0;
-    a.waves1 = pg::kTier1Waves;
+    a.waves1 = w0 / 2 > pg::kTier1Waves ? w0 / 2 : pg::kTier1Waves;  // 8 per CU: what tier 1 keeps resident
     a.big_cap = big_cap;
     a.huge_cap = huge_cap;
     a.cycles = cell_cycles;
@@ -315,7 +315,7 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.scratch = scratch;
     a.scratch_bytes = scratch_bytes;
     a.waves = w0;
-    a.waves1 = pg::kTier1Waves;
+    a.waves1 = w0 / 2 > pg::kTier1Waves ? w0 / 2 : pg::kTier1Waves;  // 8 per CU: what tier 1 keeps resident
     a.big_cap = big_cap;
     a.huge_cap = huge_cap;
     a.cycles = cell_cycles;

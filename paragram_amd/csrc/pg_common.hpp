@@ -12,7 +12,7 @@ namespace pg {
 // of F faces has 2F-4 vertices.
 constexpr int kRowCap = 48;            // tier-0 row stride (= the tier-0 plane capacity)
 constexpr int kTier1Row = 512;         // tier-1 row stride (255 planes, plus duplicates)
-constexpr int kTier1Waves = 1024;      // persistent waves of the tier-1 launch
+constexpr int kTier1Waves = 1024;      // least persistent waves of the tier-1 launch (8 per CU where more)
 constexpr int kTier2Row = 2048;        // tier-2 row stride (1023 planes, plus duplicates)
 constexpr int kTier2Waves = 256;       // persistent waves of the tier-2 launch
 

@@ -1357,9 +1357,11 @@ using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 // of 64 Morton-consecutive cells and take the flagged cells four at a time
 // (neighbours again, so the shared traversal stays coherent).  A cell that
 // outgrows the tier is queued for tier 1.
-template <typename T>
+// kFlagged: the instance that takes the cells flagged by the first one
+// (else the cells of the range, or the batch's quads).
+template <typename T, bool kFlagged>
 __device__ __forceinline__ void tier0_body(const CellParams &prm) {
-    constexpr bool kExactTier = T::kIsExact;
+    constexpr bool kExactTier = kFlagged;
     constexpr int kCellsPerWave = kWave / kT0Lanes;
     __shared__ GroupLDS<kT0Verts, kT0Planes> lds[kT0Threads / kT0Lanes];
     __shared__ CandLDS<kT0Lanes> cands[kT0Threads / kT0Lanes];
@@ -1460,8 +1462,12 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     g.flush(prm, cells);
 }
 
-__global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3))) cell_group_kernel(const CellParams prm) { tier0_body<Tier0>(prm); }
-__global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) { tier0_body<Tier0x>(prm); }
+__global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3))) cell_group_kernel(const CellParams prm) {
+    tier0_body<Tier0, false>(prm);
+}
+__global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) {
+    tier0_body<Tier0x, true>(prm);
+}
 
 // Tier 1: one cell per wave, the queued cells; rows after the tier-0 slab.
 __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
