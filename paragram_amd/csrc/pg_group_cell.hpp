@@ -1055,11 +1055,10 @@ struct Group {
         return any;
     }
 
-    // Both children of a node at once (four-cell waves): the low half of each
-    // group tests child 0, the high half child 1.
+    // Both children of a node at once: the low half of each group tests
+    // child 0, the high half child 1.
     __device__ __forceinline__ void packet_needs2(const Node &nd, bool &n0, bool &n1, float &k0, float &k1, bool &m0,
                                                   bool &m1) const {
-        static_assert(G == 16, "four cells per wave");
         constexpr int H = G / 2;
         const bool second = gl >= H;
         const float4 lo = second ? nd.lo1 : nd.lo0, hi = second ? nd.hi1 : nd.hi0;
@@ -1068,12 +1067,17 @@ struct Group {
         const uint64_t b = __ballot(need);
         m0 = ((b >> gshift) & 1ull) != 0ull;
         m1 = ((b >> (gshift + H)) & 1ull) != 0ull;
-        constexpr uint64_t kLead = 0x0001000100010001ull;  // lane 0 of each group
+        constexpr uint64_t kLead = G == 16 ? 0x0001000100010001ull : 1ull;  // lane 0 of each group
         n0 = (b & kLead) != 0ull;
         n1 = (b & (kLead << H)) != 0ull;
         const float kk = need ? k : FLT_MAX;
-        k0 = uniform(fminf(fminf(rl(kk, 0), rl(kk, 16)), fminf(rl(kk, 32), rl(kk, 48))));
-        k1 = uniform(fminf(fminf(rl(kk, H), rl(kk, 16 + H)), fminf(rl(kk, 32 + H), rl(kk, 48 + H))));
+        if constexpr (G == 16) {
+            k0 = uniform(fminf(fminf(rl(kk, 0), rl(kk, 16)), fminf(rl(kk, 32), rl(kk, 48))));
+            k1 = uniform(fminf(fminf(rl(kk, H), rl(kk, 16 + H)), fminf(rl(kk, 32 + H), rl(kk, 48 + H))));
+        } else {
+            k0 = uniform(rl(kk, 0));
+            k1 = uniform(rl(kk, H));
+        }
     }
 
     // Best-first packet traversal: the cells of a wave (Morton neighbours)
@@ -1100,7 +1104,7 @@ struct Group {
                 const Node nd = nodes[ref];
                 float k0, k1;
                 bool m0, m1, n0, n1;
-                if constexpr (G == 16) {
+                if constexpr (G == 16 || V <= 512) {  // tiers 0 and 1
                     packet_needs2(nd, n0, n1, k0, k1, m0, m1);
                 } else {
                     n0 = packet_needs(nd.lo0, nd.hi0, k0, m0);
