@@ -699,10 +699,10 @@ struct Group {
             overflow = bail = true;
             return false;
         }
+        if (!gany(mine != 0u)) return false;  // no vertex beyond: the plane does not cut
 #pragma unroll
         for (int k = 0; k < kRows; ++k) rem[(G * k) >> 6] |= gballot((mine >> k) & 1u) << ((G * k) & 63);
         const int R = mask_popc(rem);
-        if (R == 0) return false;
         if (R == mask_popc(valid)) {  // voronoi_convex_cell.cuh:430-435
             status = kEmptyCell;
 #pragma unroll
