@@ -863,7 +863,10 @@ struct Group {
     }
 
     // Prepares lane's candidate j; false when it cannot cut the cell.
-    __device__ __forceinline__ bool prepare(uint32_t j, Cand &c, bool &emptied) const {
+    // maybe_guess: j may be one of the guesses already offered (only those
+    // can already own a plane of the cell): every guess lies within the
+    // guess radius, up to the few ulps that its 3e-5 widening covers.
+    __device__ __forceinline__ bool prepare(uint32_t j, Cand &c, bool &emptied, bool &maybe_guess) const {
         const float4 p = pts[j];
         c.ax = p.x - sx;
         c.ay = p.y - sy;
@@ -876,6 +879,7 @@ struct Group {
         }
         const float a2 = c.ax * c.ax + c.ay * c.ay + c.az * c.az;
         if (a2 < guess_r2) return false;  // offered by the pre-clip; the cell has only shrunk since
+        maybe_guess = guess_r2 == 0.0f || a2 < guess_r2 * 1.00003f;
         c.H = a2 - dw;
         c.pH = a2 + fabsf(dw);
         c.pa1 = fabsf(c.ax) + fabsf(c.ay) + fabsf(c.az);
@@ -885,8 +889,8 @@ struct Group {
     // Offers a batch of up to G candidates (lane-specific j when `has`).
     __device__ __forceinline__ void offer_batch(bool has, uint32_t j) {
         Cand c;
-        bool emptied = false;
-        const bool live_c = has && prepare(j, c, emptied);
+        bool emptied = false, maybe_guess = true;
+        const bool live_c = has && prepare(j, c, emptied, maybe_guess);
         if (gany(emptied)) {
             status = kEmptyCell;
 #pragma unroll
@@ -899,7 +903,8 @@ struct Group {
         if (gl == 0 && todo) count(kStatCandidates, (uint32_t)__popcll(todo));
         if (live_c) {
             C->aH[gl] = make_float4(c.ax, c.ay, c.az, c.H);
-            C->po[gl] = make_float2(c.pH, __int_as_float(c.owner));
+            // the owner's sign bit marks a candidate that may already own a plane
+            C->po[gl] = make_float2(c.pH, __int_as_float(maybe_guess ? (c.owner | INT32_MIN) : c.owner));
         }
         float hkey = live_c ? c.H : FLT_MAX;
         while (todo) {
@@ -920,8 +925,9 @@ struct Group {
             q.pH = po.x;
             q.pa1 = fabsf(q.ax) + fabsf(q.ay) + fabsf(q.az);
             q.c2 = fmaf(kG32, q.pH, kTiny32);
-            q.owner = __float_as_int(po.y);
-            if (bloom && has_plane(q.owner)) continue;
+            const int ow = __float_as_int(po.y);
+            q.owner = ow & INT32_MAX;
+            if (ow < 0 && bloom && has_plane(q.owner)) continue;
             if (clip(q)) {
                 if (gl == 0) count(kStatClips);
                 if (status != kSuccess) return;
