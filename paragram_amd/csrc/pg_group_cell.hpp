@@ -546,7 +546,9 @@ struct Group {
         const float dw = lo.w - sw;
         const float hlow = d2 - dw;
         key = hlow;
-        if (!bounded) return false;
+        // a box that holds the seed meets every vertex's ball (each passes
+        // through the seed), so neither test below can prune it
+        if (!bounded || d2 == 0.0f) return false;
         // the octants a box overlaps form a product set, so the largest
         // corner radius over them is a sum of per-axis maxima
         const float rho2 = (fmaxf(alx <= 0.0f ? Lq_[0] : 0.0f, ahx >= 0.0f ? Uq_[0] : 0.0f) +
