@@ -434,8 +434,10 @@ struct Group {
         float4 pos = make_float4(0.f, 0.f, 0.f, -1.0f);
         const bool ok = D > 2.0f * eD && pD > kTiny32;
         if (ok) {
-            const float inv = 1.0f / D;
-            const float den = (1.0f + kSlack) / (D - eD);
+            // hardware reciprocals (1 ulp): their error, and the products',
+            // lie far inside the kSlack = 1e-6 relative terms of the bound
+            const float inv = __builtin_amdgcn_rcpf(D);
+            const float den = (1.0f + kSlack) * __builtin_amdgcn_rcpf(D - eD);
             const float x = Nx * inv, y = Ny * inv, z = Nz * inv;
             const float ex = (kG32 * pNx + fabsf(x) * eD) * den + fabsf(x) * kSlack;
             const float ey = (kG32 * pNy + fabsf(y) * eD) * den + fabsf(y) * kSlack;
