@@ -10,7 +10,7 @@
 //           about 34 faces, e.g. near-range LiDAR returns with 200+ faces);
 //   tier 2: G = 64 lanes, 1024 vertex slots, 1023 plane slots, 10-bit plane
 //           indices (one cell per wave and block) -- the rare cells that
-//           outgrow tier 1 (or its 256-entry traversal list).
+//           outgrow tier 1.
 //
 // Per cell in LDS:
 //   vpos[V]  float4  vertex position y = N/D in seed-relative doubled
@@ -305,7 +305,6 @@ struct Group {
     static constexpr int kLS = L / kWave;  // wait-list slots per lane
     GroupLDS<V, P> *S;
     CandLDS<G> *C;
-    float lk[kLS];
     uint32_t lr[kLS];
     int gl;      // lane within the group
     int gshift;  // bit offset of the group in a wave ballot
@@ -997,59 +996,21 @@ struct Group {
     }
 
     // ---------------------------------------------------------- wait list
-    // Best-first wait list shared by the cells of one wave, in registers:
-    // entry t sits in slot t / 64 of lane t % 64.  Entries are
-    // parent << 1 | child, keyed by the child's bound.
-    __device__ __forceinline__ void list_put(int t, uint32_t e, float key) {
+    // Depth-first wait list (a stack) shared by the cells of one wave, in
+    // registers: entry t sits in slot t / 64 of lane t % 64.  Entries are
+    // parent << 1 | child.
+    __device__ __forceinline__ void list_push(int t, uint32_t e) {
         const bool me = lane64 == (t & (kWave - 1));
         const int sl = t / kWave;
 #pragma unroll
-        for (int j = 0; j < kLS; ++j) {
-            const bool w = me && j == sl;
-            lr[j] = w ? e : lr[j];
-            lk[j] = w ? key : lk[j];
-        }
+        for (int j = 0; j < kLS; ++j) lr[j] = (me && j == sl) ? e : lr[j];
     }
-
-    // Smallest-key entry (wave arg-min), -1 if the list is empty.
-    __device__ __forceinline__ int list_argmin(int sz, float &kmin) const {
-        kmin = FLT_MAX;
-        if (sz == 0) return -1;
-        float k = FLT_MAX;
-        int idx = -1;
+    __device__ __forceinline__ uint32_t list_pop(int &sz) {
+        const int t = --sz;
+        uint32_t r = 0u;
 #pragma unroll
-        for (int j = 0; j < kLS; ++j) {
-            const int t = lane64 + kWave * j;
-            if (t < sz && (idx < 0 || lk[j] < k)) {
-                k = lk[j];
-                idx = t;
-            }
-        }
-        const float r = row_min(k);
-        const float m = uniform(fminf(fminf(rl(r, 0), rl(r, 16)), fminf(rl(r, 32), rl(r, 48))));
-        const uint64_t hit = __ballot(idx >= 0 && k == m);
-        kmin = m;
-        if (!hit) return sz - 1;  // unordered keys: take the last entry
-        return rl(idx, (int)__builtin_ctzll(hit));
-    }
-
-    // Removes entry im (the last entry moves into its place); returns it.
-    __device__ __forceinline__ uint32_t list_take(int &sz, int im) {
-        const int last = sz - 1;
-        uint32_t re = 0u, rlast = 0u;
-        float klast = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kLS; ++j) {
-            re = j == im / kWave ? lr[j] : re;
-            rlast = j == last / kWave ? lr[j] : rlast;
-            klast = j == last / kWave ? lk[j] : klast;
-        }
-        const uint32_t e = rl(re, im & (kWave - 1));
-        const uint32_t el = rl(rlast, last & (kWave - 1));
-        const float kl = rl(klast, last & (kWave - 1));
-        list_put(im, el, kl);
-        sz = last;
-        return e;
+        for (int j = 0; j < kLS; ++j) r = j == t / kWave ? lr[j] : r;
+        return rl(r, t & (kWave - 1));
     }
 
     // Wave-wide: does any cell of the wave still need this box?  key: the
@@ -1090,17 +1051,17 @@ struct Group {
         }
     }
 
-    // Best-first packet traversal: the cells of a wave (Morton neighbours)
+    // Depth-first packet traversal: the cells of a wave (Morton neighbours)
     // walk the hierarchy together.  A box is expanded while any of them may
-    // still be cut by a point in it; its key is the smallest lower bound of
-    // H over the cells that need it; leaves are offered to every cell, each
-    // filtering and clipping on its own lanes.  Unpruned children wait in one
-    // wave-wide keyed list and the wave always goes on with the smallest key
-    // (the order voronoi_bvh.cuh:246-283 keeps with its min-extraction pops),
-    // re-testing a waiting box when it comes out.  A leaf goes only to the
-    // cells whose own test kept its box (the others could not be cut by its
-    // points).  A full list sends the wave's cells to the next tier instead
-    // of dropping entries.
+    // still be cut by a point in it; of two needed children the one with the
+    // smaller key (the smallest lower bound of H over the cells that need
+    // it) comes first and the other waits on a wave-wide stack, re-tested
+    // when it comes out.  A leaf goes only to the cells whose own test kept
+    // its box (the others could not be cut by its points).  The reference
+    // pops the smallest key instead (voronoi_bvh.cuh:246-283); after the kNN
+    // pre-clip the order changes little of the work and the stack needs no
+    // arg-min, and the diagram does not depend on it.  A full stack sends the
+    // wave's cells to the next tier instead of dropping entries.
     __device__ __forceinline__ void traverse() {
         uint32_t ref = uniform(root_ref);
         int sz = 0;
@@ -1123,34 +1084,21 @@ struct Group {
                 const uint32_t r0 = uniform(__float_as_uint(nd.hi0.w)), r1 = uniform(__float_as_uint(nd.hi1.w));
                 if (n0 || n1) {
                     const bool near0 = !n1 || (n0 && k0 <= k1);
-                    const float knear = near0 ? k0 : k1;
                     if (n0 && n1) {  // the farther child waits
                         if (sz >= L) {
                             overflow = true;
                             return;
                         }
-                        list_put(sz++, (ref << 1) | (near0 ? 1u : 0u), near0 ? k1 : k0);
+                        list_push(sz++, (ref << 1) | (near0 ? 1u : 0u));
                     }
-                    float kmin;
-                    const int im = list_argmin(sz, kmin);
-                    if (im < 0 || knear <= kmin) {
-                        ref = near0 ? r0 : r1;
-                        mine = near0 ? m0 : m1;
-                        have = true;
-                    } else {  // a waiting box is nearer: the near child waits too
-                        if (sz >= L) {
-                            overflow = true;
-                            return;
-                        }
-                        list_put(sz++, (ref << 1) | (near0 ? 0u : 1u), knear);
-                    }
+                    ref = near0 ? r0 : r1;
+                    mine = near0 ? m0 : m1;
+                    have = true;
                 }
             }
             while (!have) {
-                float kmin;
-                const int im = list_argmin(sz, kmin);
-                if (im < 0) return;
-                const uint32_t e = list_take(sz, im);
+                if (sz == 0) return;
+                const uint32_t e = list_pop(sz);
                 const uint32_t parent = e >> 1, c = e & 1u;
                 const Node nd = nodes[parent];
                 float k;
@@ -1307,7 +1255,7 @@ struct Group {
         S = s;
         C = c;
 #pragma unroll
-        for (int j = 0; j < kLS; ++j) lk[j] = 0.0f, lr[j] = 0u;
+        for (int j = 0; j < kLS; ++j) lr[j] = 0u;
         gl = lane & (G - 1);
         lane64 = lane;
         gshift = grp * G;
@@ -1356,8 +1304,10 @@ __device__ __forceinline__ int batch_find(const int64_t *off, int n_clouds, int6
 
 // ------------------------------------------------------------------ tiers
 constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Threads = 256;
-constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 256, kT1Threads = 128;
-constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 1024, kT2Threads = 64;
+// (a depth-first stack holds at most one entry per level of the hierarchy,
+// fewer than 64 levels: 128 entries never fill)
+constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 128, kT1Threads = 128;
+constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 128, kT2Threads = 64;
 static_assert(kT0Planes <= kRowCap && kT1Planes <= kTier1Row && kT2Planes <= kTier2Row, "rows fit their strides");
 using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, false>;
 using Tier0x = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, true>;
