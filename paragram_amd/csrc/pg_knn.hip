@@ -22,6 +22,10 @@ namespace pg {
 constexpr int kKnnThreads = 128;
 constexpr int kKnnStack = 64;      // deep pass: never overflows (see above)
 constexpr int kKnnStackFast = 32;  // first pass: 16 KB of LDS per block instead of 32 KB
+#ifndef PG_KNN_LOADS
+#define PG_KNN_LOADS 2
+#endif
+constexpr int kLoads = PG_KNN_LOADS;  // leaf points loaded ahead
 
 __device__ __forceinline__ float box_dist2(float4 lo, float4 hi, float qx, float qy, float qz) {
     const float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
@@ -35,11 +39,11 @@ __device__ __forceinline__ float box_dist2(float4 lo, float4 hi, float qx, float
 // and the deep pass (kDeep, 64 entries) redoes it, so no query's result
 // depends on the stack size.
 template <int K, int S, bool kDeep>
-__global__ void __launch_bounds__(kKnnThreads)
-    knn_kernel(int64_t n, IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
-               const uint32_t *__restrict__ perm, const Node *__restrict__ nodes, int k_out,
-               int32_t *__restrict__ out_ids, float *__restrict__ out_d2, const IndexHeader *__restrict__ clouds,
-               const int64_t *__restrict__ cloud_cells, int n_clouds, int sorted_rows, uint32_t *__restrict__ spill) {
+__device__ __forceinline__ void knn_body(int64_t n, IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
+                                         const uint32_t *__restrict__ perm, const Node *__restrict__ nodes, int k_out,
+                                         int32_t *__restrict__ out_ids, float *__restrict__ out_d2,
+                                         const IndexHeader *__restrict__ clouds, const int64_t *__restrict__ cloud_cells,
+                                         int n_clouds, int sorted_rows, uint32_t *__restrict__ spill) {
     __shared__ uint32_t stack[S][kKnnThreads];
     int64_t i;
     if constexpr (kDeep) {
@@ -93,13 +97,13 @@ __global__ void __launch_bounds__(kKnnThreads)
     while (true) {
         if (is_leaf(ref)) {
             const uint32_t f = leaf_first(ref), c = leaf_count(ref);
-            // four loads in flight, then four offers (leaf order kept)
-            for (uint32_t t = 0; t < c; t += 4) {
-                float4 p[4];
+            // kLoads loads in flight, then as many offers (leaf order kept)
+            for (uint32_t t = 0; t < c; t += kLoads) {
+                float4 p[kLoads];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) p[u] = pts[f + min(t + (uint32_t)u, c - 1u)];
+                for (int u = 0; u < kLoads; ++u) p[u] = pts[f + min(t + (uint32_t)u, c - 1u)];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kLoads; ++u) {
                     const uint32_t j = f + t + (uint32_t)u;
                     if (t + (uint32_t)u < c && (int64_t)j != i) offer(p[u], j);
                 }
@@ -165,6 +169,25 @@ __global__ void __launch_bounds__(kKnnThreads)
     }
 }
 
+#define PG_KNN_ARGS                                                                                                 \
+    int64_t n, IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts, const uint32_t *__restrict__ perm,    \
+        const Node *__restrict__ nodes, int k_out, int32_t *__restrict__ out_ids, float *__restrict__ out_d2,       \
+        const IndexHeader *__restrict__ clouds, const int64_t *__restrict__ cloud_cells, int n_clouds,               \
+        int sorted_rows, uint32_t *__restrict__ spill
+// first pass: 128 VGPRs and 16 KB of LDS per block, four waves per SIMD
+template <int K>
+__global__ void __launch_bounds__(kKnnThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+    knn_kernel(PG_KNN_ARGS) {
+    knn_body<K, kKnnStackFast, false>(n, hdr, pts, perm, nodes, k_out, out_ids, out_d2, clouds, cloud_cells, n_clouds,
+                                      sorted_rows, spill);
+}
+template <int K>
+__global__ void __launch_bounds__(kKnnThreads) knn_deep_kernel(PG_KNN_ARGS) {
+    knn_body<K, kKnnStack, true>(n, hdr, pts, perm, nodes, k_out, out_ids, out_d2, clouds, cloud_cells, n_clouds,
+                                 sorted_rows, spill);
+}
+#undef PG_KNN_ARGS
+
 __global__ void knn_reset_kernel(IndexHeader *hdr) { hdr->knn_spill = 0u; }
 
 template <int K>
@@ -173,12 +196,12 @@ static void launch_knn(int64_t n, IndexHeader *hdr, const float4 *pts, const uin
                        int sorted, uint32_t *spill, hipStream_t stream) {
     const unsigned grid = (unsigned)((n + kKnnThreads - 1) / kKnnThreads);
     hipLaunchKernelGGL(knn_reset_kernel, dim3(1), dim3(1), 0, stream, hdr);
-    hipLaunchKernelGGL((knn_kernel<K, kKnnStackFast, false>), dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts,
-                       perm, nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted, spill);
+    hipLaunchKernelGGL(knn_kernel<K>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k, ids, d2,
+                       clouds, cloud_cells, n_clouds, sorted, spill);
     // the deep pass reads the spill count on the device; its grid covers
     // every query in case all of them spilled (blocks past the count exit)
-    hipLaunchKernelGGL((knn_kernel<K, kKnnStack, true>), dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm,
-                       nodes, k, ids, d2, clouds, cloud_cells, n_clouds, sorted, spill);
+    hipLaunchKernelGGL(knn_deep_kernel<K>, dim3(grid), dim3(kKnnThreads), 0, stream, n, hdr, pts, perm, nodes, k, ids,
+                       d2, clouds, cloud_cells, n_clouds, sorted, spill);
 }
 
 hipError_t knn(int64_t n, void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream, int n_clouds,
