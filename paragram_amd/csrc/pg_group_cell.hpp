@@ -1321,6 +1321,13 @@ using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 // of 64 Morton-consecutive cells and take the flagged cells four at a time
 // (neighbours again, so the shared traversal stays coherent).  A cell that
 // outgrows the tier is queued for tier 1.
+// The address of the launch's parameters, opaque to the optimiser: fields
+// read through it are loaded where they are used.
+__device__ __forceinline__ const CellParams *reread(const CellParams *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // kFlagged: the instance that takes the cells flagged by the first one
 // (else the cells of the range, or the batch's quads).
 template <typename T, bool kFlagged>
@@ -1341,35 +1348,38 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         const uint64_t t0 = clock64();
         g.active = active;
         const uint32_t cell = (uint32_t)(prm.begin + (active ? c : 0));
-        if (active) g.start(cell, prm);
+        if (active) g.start(cell, *reread(&prm));
         g.traverse();
         if (!active) return;
-        const uint32_t orig = prm.perm[cell];
+        // the launch's fields again from the argument segment (scalar loads)
+        // rather than kept in registers across the traversal
+        const CellParams &P = *reread(&prm);
+        const uint32_t orig = P.perm[cell];
         bool done = false;
         // a cell built inside a predicted cube stands only if the cube cut nothing
         if (!kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
             g.overflow = g.bail = true;
         if (!g.overflow) {
-            if (g.gl == 0) prm.row_offsets[cell] = c * prm.row_stride;
-            done = g.emit(prm, orig, prm.rows + c * prm.row_stride, prm.row_stride);
+            if (g.gl == 0) P.row_offsets[cell] = c * P.row_stride;
+            done = g.emit(P, orig, P.rows + c * P.row_stride, P.row_stride);
         }
         if (!done && g.gl == 0) {
             if (!kExactTier && g.bail) {  // flag for the FP64-capable instance
-                prm.exact_list[c] = 1u;
-                atomicAdd(&prm.counters[5], 1u);
+                P.exact_list[c] = 1u;
+                atomicAdd(&P.counters[5], 1u);
             } else {  // queue for the large-capacity tier
                 g.count(kStatOverflow);
-                const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
-                if (slot < prm.big_cap) {
-                    prm.big_list[slot] = cell;
+                const uint32_t slot = atomicAdd(&P.counters[1], 1u);
+                if (slot < P.big_cap) {
+                    P.big_list[slot] = cell;
                 } else {
-                    prm.row_offsets[cell] = c * prm.row_stride;
-                    prm.counts[orig] = 0;
-                    prm.status[orig] = kPlaneOverflow;
+                    P.row_offsets[cell] = c * P.row_stride;
+                    P.counts[orig] = 0;
+                    P.status[orig] = kPlaneOverflow;
                 }
             }
         }
-        if (prm.cycles && g.gl == 0) prm.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (P.cycles && g.gl == 0) P.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
         if (g.gl == 0 && !g.bail) cells++;
     };
     if constexpr (!kExactTier) {
