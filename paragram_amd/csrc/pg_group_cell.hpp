@@ -1460,25 +1460,26 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         const uint32_t cell = prm.big_list[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
-        g.start(cell, prm);
+        g.start(cell, *reread(&prm));
         g.traverse();
-        const uint32_t orig = prm.perm[cell];
-        const int64_t off = (int64_t)(prm.end - prm.begin) * prm.row_stride + (int64_t)k * kTier1Row;
-        if (lane == 0) prm.row_offsets[cell] = off;
-        if (g.overflow || !g.emit(prm, orig, prm.rows + off, kTier1Row)) {
+        const CellParams &P = *reread(&prm);
+        const uint32_t orig = P.perm[cell];
+        const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)k * kTier1Row;
+        if (lane == 0) P.row_offsets[cell] = off;
+        if (g.overflow || !g.emit(P, orig, P.rows + off, kTier1Row)) {
             if (lane == 0) {  // queue for tier 2
                 g.count(kStatOverflow);
-                const uint32_t slot = atomicAdd(&prm.counters[3], 1u);
-                if (slot < prm.huge_cap) {
-                    prm.huge_list[slot] = cell;
+                const uint32_t slot = atomicAdd(&P.counters[3], 1u);
+                if (slot < P.huge_cap) {
+                    P.huge_list[slot] = cell;
                 } else {
-                    prm.counts[orig] = 0;
-                    prm.status[orig] = kPlaneOverflow;
+                    P.counts[orig] = 0;
+                    P.status[orig] = kPlaneOverflow;
                 }
             }
         }
-        if (prm.cycles && lane == 0)
-            prm.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (P.cycles && lane == 0)
+            P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
     }
     g.flush(prm, 0);
 }
@@ -1501,23 +1502,24 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
         const uint32_t cell = prm.huge_list[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
-        g.start(cell, prm);
+        g.start(cell, *reread(&prm));
         g.traverse();
-        const uint32_t orig = prm.perm[cell];
-        const int64_t off = (int64_t)(prm.end - prm.begin) * prm.row_stride + (int64_t)prm.big_cap * kTier1Row +
+        const CellParams &P = *reread(&prm);
+        const uint32_t orig = P.perm[cell];
+        const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)P.big_cap * kTier1Row +
                             (int64_t)k * kTier2Row;
-        if (lane == 0) prm.row_offsets[cell] = off;
-        if (g.overflow || !g.emit(prm, orig, prm.rows + off, kTier2Row)) {
+        if (lane == 0) P.row_offsets[cell] = off;
+        if (g.overflow || !g.emit(P, orig, P.rows + off, kTier2Row)) {
             // beyond every tier: report the reference's capacity status with
             // an empty row (counted, never silent)
             if (lane == 0) {
                 g.count(kStatOverflow);
-                prm.counts[orig] = 0;
-                prm.status[orig] = kPlaneOverflow;
+                P.counts[orig] = 0;
+                P.status[orig] = kPlaneOverflow;
             }
         }
-        if (prm.cycles && lane == 0)
-            prm.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (P.cycles && lane == 0)
+            P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
     }
     g.flush(prm, 0);
 }
