@@ -1321,8 +1321,11 @@ using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 // of 64 Morton-consecutive cells and take the flagged cells four at a time
 // (neighbours again, so the shared traversal stays coherent).  A cell that
 // outgrows the tier is queued for tier 1.
-// The address of the launch's parameters, opaque to the optimiser: fields
-// read through it are loaded where they are used.
+// The address of the block's copy of the launch's parameters (in LDS: one
+// copy per block, where a kernel argument whose address is taken would be
+// copied per lane to scratch), opaque to the optimiser: fields read through
+// it are loaded where they are used, not held in registers across the
+// traversal.
 __device__ __forceinline__ const CellParams *reread(const CellParams *p) {
     asm volatile("" : "+s"(p));
     return p;
@@ -1336,6 +1339,9 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     constexpr int kCellsPerWave = kWave / kT0Lanes;
     __shared__ GroupLDS<kT0Verts, kT0Planes> lds[kT0Threads / kT0Lanes];
     __shared__ CandLDS<kT0Lanes> cands[kT0Threads / kT0Lanes];
+    __shared__ CellParams sprm;
+    if (threadIdx.x == 0) sprm = prm;
+    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int grp = lane / kT0Lanes;
     if (kExactTier && prm.counters[5] == 0u) return;  // no cell was flagged
@@ -1348,12 +1354,12 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         const uint64_t t0 = clock64();
         g.active = active;
         const uint32_t cell = (uint32_t)(prm.begin + (active ? c : 0));
-        if (active) g.start(cell, *reread(&prm));
+        if (active) g.start(cell, *reread(&sprm));
         g.traverse();
         if (!active) return;
         // the launch's fields again from the argument segment (scalar loads)
         // rather than kept in registers across the traversal
-        const CellParams &P = *reread(&prm);
+        const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
         bool done = false;
         // a cell built inside a predicted cube stands only if the cube cut nothing
@@ -1447,6 +1453,9 @@ __global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const Cell
 __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT1Verts, kT1Planes> lds[kT1Threads / kWave];
     __shared__ CandLDS<kT1Lanes> cands[kT1Threads / kWave];
+    __shared__ CellParams sprm;
+    if (threadIdx.x == 0) sprm = prm;
+    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     Tier1 g;
     g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
@@ -1460,9 +1469,9 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         const uint32_t cell = prm.big_list[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
-        g.start(cell, *reread(&prm));
+        g.start(cell, *reread(&sprm));
         g.traverse();
-        const CellParams &P = *reread(&prm);
+        const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
         const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)k * kTier1Row;
         if (lane == 0) P.row_offsets[cell] = off;
@@ -1489,6 +1498,9 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
 __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT2Verts, kT2Planes> lds;
     __shared__ CandLDS<kT2Lanes> cands;
+    __shared__ CellParams sprm;
+    if (threadIdx.x == 0) sprm = prm;
+    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     Tier2 g;
     g.setup(prm, &lds, &cands);
@@ -1502,9 +1514,9 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
         const uint32_t cell = prm.huge_list[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
-        g.start(cell, *reread(&prm));
+        g.start(cell, *reread(&sprm));
         g.traverse();
-        const CellParams &P = *reread(&prm);
+        const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
         const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)P.big_cap * kTier1Row +
                             (int64_t)k * kTier2Row;
