@@ -138,8 +138,8 @@ def main() -> int:
         print(json.dumps(rec))
         return 0
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is "
-              "described in DESIGN.md and not built)" % args.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
+              "designed but not built)" % args.gpus, file=sys.stderr)
         return 2
 
     import numpy as np

@@ -7,7 +7,7 @@ for new coordinates of the same shape: one graph launch and one
 device-to-host read (the edge count) per diagram, no allocation.  The rare
 union pass (an asymmetric edge) runs eagerly after the replay.
 
-No counterpart in the reference (SURVEY's batched / hipGraph item); the
+No counterpart in the reference (its calls are eager); the
 result is that of ``voronoi_diagram`` / ``power_diagram`` on the same input.
 Buffers are cleared by the library's own fill kernel inside the graph (no
 memset nodes).

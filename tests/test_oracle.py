@@ -1,6 +1,6 @@
 """Pins the CPU oracle against what the mathematics fixes.
 
-The reference ships no tests or golden vectors (SURVEY: none exist), so the
+The reference ships no tests or golden vectors, so the
 oracle is checked against independent constructions:
 
 * Qhull (scipy): the Voronoi adjacency of points clipped to a box equals the
