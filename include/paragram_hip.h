@@ -34,23 +34,25 @@
 extern "C" {
 #endif
 
-#define PG_ABI_VERSION 1
+#define PG_ABI_VERSION 2
 
 /* Number of statistics counters written by the calls that take `stats`
- * (uint64 each): 0 FP64 rechecks, 1 unresolved, 2 inconsistent,
- * 3 capacity overflows, 4 clips, 5 candidates tested, 6 nodes visited,
+ * (uint64 each, accumulated): 0 FP64 rechecks, 1 removal tests undecided
+ * even by expansion arithmetic, 2 inconsistent, 3 cells promoted to a
+ * larger-capacity tier, 4 clips, 5 candidates tested, 6 nodes visited,
  * 7 cells, 8 asymmetric edges, 9 entries to empty cells, 10 leaves visited,
- * 11 candidates rejected by the support test, 15 edge total E. */
-#define PG_NUM_STATS 16
+ * 11 candidates rejected by the support test, 12 cells truncated (beyond the
+ * largest tier: status 1 and an empty row), 13 rows that did not fit the row
+ * pool, 14 (a maximum, not a sum) the pool elements such a launch needed,
+ * 15 edge total E, 16 removal tests decided by expansion arithmetic,
+ * 17..23 original ids (plus 1) of the first cells with such a test. */
+#define PG_NUM_STATS 24
 
 /* ABI version compiled into the library (== PG_ABI_VERSION). */
 int pg_abi_version(void);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *pg_last_error(void);
-
-/* Row stride of the capacity-0 tier (rows of the rare larger cells live after it). */
-int pg_row_capacity(void);
 
 /* Bytes of the spatial-index workspace for n points.
  * Replaces the allocations of pwr_bvh.cu:228-270 (cudaMalloc of boxes,
@@ -68,6 +70,22 @@ int pg_index_workspace_bytes(int64_t n, uint64_t *bytes);
 int pg_build_index(int64_t n, const float *points, const float *weights, int32_t leaf_size, void *workspace,
                    uint64_t workspace_bytes, void *stream);
 
+/* Byte offsets within the index workspace of n points (host only, no device
+ * work): offsets[0..5] = header, Morton-sorted float4 points (x, y, z, w),
+ * perm (uint32 sorted -> original), inv (uint32 original -> sorted), nodes
+ * (64-byte records: child 0 lo.xyz + max weight, hi.xyz + reference, then
+ * child 1 the same; a reference with bit 31 set is a leaf,
+ * first << 5 | (count - 1), else an internal node), total.  For inspecting
+ * and testing the hierarchy. */
+int pg_index_layout(int64_t n, uint64_t *offsets);
+
+/* Leaf statistics of the index of pg_build_index (or of a batch):
+ * out[0..3] = leaves, fewest points in a leaf, most, points in all leaves
+ * (device memory, 4 uint64).
+ * Replaces print_leaf_statistics's device pass (pwr_bvh.cu:341-453), printed
+ * by the debug output of a power diagram (laguerre_ultra.cu:259-265). */
+int pg_leaf_stats(int64_t n, const void *workspace, uint64_t *out, void *stream);
+
 /* The k nearest neighbours (1 <= k <= 16) of every point over the index of
  * pg_build_index: ids[i*k + t] (original ids, ascending squared distance,
  * the point itself excluded, a short list padded with its last id or 0) and,
@@ -80,19 +98,27 @@ int pg_knn(int64_t n, void *workspace, int32_t k, int32_t *ids, float *d2, void 
 /* Persistent waves used by pg_compute_cells on the current device. */
 int pg_default_waves(int32_t *waves);
 
-/* Bytes of cell scratch for `waves` persistent waves and launches over at
- * most max_cells cells (work counters and the queues of the two
- * large-capacity tiers; the cells themselves live in LDS).
+/* Bytes of cell scratch for launches over at most max_cells cells (work
+ * counters, the row-pool counter, and per cell the FP64 flag and the
+ * queues of the two large-capacity tiers: 12 bytes; the cells themselves
+ * live in LDS).
  * Replaces ConvexCellMemory::required_bytes (voronoi_convex_cell.cuh:37-44). */
-int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes);
+int pg_cell_scratch_bytes(int64_t max_cells, uint64_t *bytes);
 
-/* int32 elements of the row buffer of a launch over `cells` cells. */
-int pg_row_buffer_elems(int64_t cells, uint64_t *elems);
+/* Default int32 elements of the row pool of a launch over `cells` cells
+ * (32 per cell + 65536). */
+int pg_row_pool_elems(int64_t cells, uint64_t *elems);
 
 /* Builds the cells of the Morton-sorted range [begin, end): each cell's
  * sorted neighbour row (original ids) goes to rows + row_offsets[i] (i the
  * sorted id), its row length to counts[orig] and its status to status[orig].
- * rows holds pg_row_buffer_elems(end - begin) elements.
+ * rows is a pool of rows_elems int32 from which every cell takes exactly its
+ * row length.  A row that does not fit is counted in stats[13] with the
+ * pool size the launch needs in stats[14] (its cell gets status 1 and an
+ * empty row): the caller then runs the launch again with a pool that large.
+ * No other capacity can fill: a cell that outgrows a tier is rebuilt by the
+ * next (stats[3]), and only a cell beyond the largest (1023 planes, 1024
+ * vertices) gets status 1 (stats[12]).
  * guesses / guess_offsets: optional CSR of candidate ids clipped first
  * (original ids, or sorted ids when guesses_sorted != 0); negative ids skip.
  * seed_window (0..16): without guesses, that many Morton-adjacent points are
@@ -105,9 +131,8 @@ int pg_row_buffer_elems(int64_t cells, uint64_t *elems);
  * (voronoi_ultra.cu:39-141,283-305; laguerre_ultra.cu:43-174,337-357). */
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                      const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window, int32_t *rows,
-                     int64_t *row_offsets,
-                     int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                     int32_t waves, uint32_t *cell_cycles, void *stream);
+                     uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
+                     void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream);
 
 /* The library's own kNN pre-clip: row i (Morton-sorted order) of ids holds
  * the sorted indices of point i's k nearest neighbours (itself as padding);
@@ -139,9 +164,9 @@ int pg_knn_guesses(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int3
 int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                         const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
                         const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
-                        int32_t *rows, int64_t *row_offsets,
-                        int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                        int32_t waves, uint32_t *cell_cycles, void *stream);
+                        int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status,
+                        uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
+                        uint32_t *cell_cycles, void *stream);
 
 /* ---- Learned cell radius -------------------------------------------------
  * No counterpart in the reference (its traversal starts from the radius of a
@@ -206,15 +231,38 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
  * pg_knn_guesses (guess_stride = k, guess_d2 = its kth_d2 or null). */
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
                            int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t guess_stride, const float *guess_d2, int32_t *rows, int64_t *row_offsets,
-                           int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                           int32_t waves, uint32_t *cell_cycles, void *stream);
+                           int32_t guess_stride, const float *guess_d2, int32_t *rows, uint64_t rows_elems,
+                           int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
+                           uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream);
 
 /* pg_knn over a batch index: each point's neighbours come from its own cloud
  * (the batch's ids are global); a point alone in its cloud gets its own id as
  * padding (the cell kernels skip it). */
 int pg_knn_batch(int64_t n, void *workspace, int32_t n_clouds, int32_t k, int32_t *ids, float *d2,
                  void *stream);
+
+/* Exact signs of m removal predicates on the GPU, over the index of
+ * pg_build_index (its points and FP32 box).  queries: m x 5 int32 (seed,
+ * three planes of a vertex, query plane): ids >= 0 are original point ids,
+ * ids < 0 box planes -1 - (2 axis + side) (side 0 = low).  out: m x 3 int32
+ * (sign(P) sign(D), sign(D), sign(P)) with D = det[a_0; a_1; a_2] and
+ * P = a_q . N - H_q D in the seed's doubled coordinates; (2, 0, 0) when an
+ * expansion exceeded its capacity, (3, 0, 0) for an id out of range.  The
+ * same device code decides the removal tests the FP64 filter cannot
+ * (expansion arithmetic; Shewchuk, DCG 18, 1997).
+ * Replaces the fast-math FP32 test of CCVertexIsInHalfSpace
+ * (voronoi_convex_cell.cuh:256-260, faster_convex_cell.cuh:280-285) where
+ * it is ambiguous. */
+int pg_exact_side(int64_t n, const void *workspace, int64_t m, const int32_t *queries, int32_t *out, void *stream);
+
+/* The same exact-sign code run on the host for one predicate (host memory;
+ * no GPU needed): seed (x, y, z, w), planes 4 x (x, y, z, w) (three vertex
+ * planes, then the query plane), box_codes[4] (-1: the plane of that point,
+ * 0..5: box plane 2 axis + side of [box_lo, box_hi], point ignored); out as
+ * pg_exact_side.  For checking the device code against rational arithmetic
+ * on machines without a GPU. */
+int pg_exact_side_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
+                       const float *box_hi, int32_t *out);
 
 /* Bytes of temporary storage for pg_scan_counts over n values. */
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes);

@@ -4,17 +4,23 @@ Mirrors the seam of the reference (paragram/_backend.py:138-195): ``_C`` is an
 object with ``build_voronoi_ultra`` and ``build_laguerre_voronoi_ultra``, or
 ``None`` when the engine cannot be loaded, in which case importing
 ``paragram_amd.cuda`` raises ``ImportError`` (paragram/cuda/__init__.py:3-4).
-The engine is the in-tree ``libparagram_hip.so`` (built by
-``paragram_amd._build``); there is no CPU fallback.
+The engine is the in-tree ``libparagram_hip.so``.  As the reference compiles
+its extension on first import when no build exists (_backend.py:147-190),
+a missing library is built here on first import with hipcc for gfx950
+(``paragram_amd._build``: ``MAX_JOBS`` (default 10), ``FAST_COMPILE``,
+``DEBUG_CUDA``, ``PROFILE_CUDA``, ``VERBOSE``), unless
+``PARAGRAM_NO_BUILD=1``; without hipcc the engine is unavailable.  There is
+no CPU fallback.
 """
 from __future__ import annotations
 
 import os
+import time
 from typing import Optional
 
 import torch
 
-from . import _engine
+from . import _build, _engine
 
 _load_error: Optional[str] = None
 
@@ -38,13 +44,42 @@ class _HipBackend:
                                      initial_guesses_offsets, leaf_size)
 
 
-def _load() -> Optional[_HipBackend]:
+def _build_missing(lib_path: str) -> bool:
+    """Builds the library when it is missing; False (and the reason in
+    _load_error) when it cannot be built here."""
     global _load_error
-    if not os.path.exists(_engine.LIB_PATH):
-        _load_error = f"{_engine.LIB_PATH} is missing (build it with paragram_amd._build.build())"
+    if os.environ.get("PARAGRAM_NO_BUILD") == "1":
+        _load_error = f"{lib_path} is missing and PARAGRAM_NO_BUILD=1"
+        return False
+    try:
+        _build._hipcc()
+    except RuntimeError as exc:
+        _load_error = f"{lib_path} is missing and cannot be built: {exc}"
+        print(f"paragram_amd: {_load_error}; the HIP engine is unavailable.", flush=True)
+        return False
+    jobs = os.environ.get("MAX_JOBS") or "10"
+    print(f"paragram_amd: building the HIP library for gfx950 with MAX_JOBS={jobs}", flush=True)
+    t0 = time.time()
+    prev = os.environ.get("MAX_JOBS")
+    os.environ["MAX_JOBS"] = jobs
+    try:
+        _build.build(lib_path=lib_path)
+    except RuntimeError as exc:
+        _load_error = f"building {lib_path} failed: {exc}"
+        return False
+    finally:
+        if prev is None:
+            os.environ.pop("MAX_JOBS", None)
+    print(f"paragram_amd: HIP library ready in {time.time() - t0:.2f} seconds.", flush=True)
+    return True
+
+
+def _load(lib_path: str = _engine.LIB_PATH) -> Optional[_HipBackend]:
+    global _load_error
+    if not os.path.exists(lib_path) and not _build_missing(lib_path):
         return None
     try:
-        return _HipBackend(_engine.HipLibrary(_engine.LIB_PATH))
+        return _HipBackend(_engine.HipLibrary(lib_path))
     except OSError as exc:
         _load_error = str(exc)
         return None

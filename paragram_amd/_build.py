@@ -20,6 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libparagram_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
+CPU_LIB_PATH = os.path.join(HERE, "libparagram_cpu.so")
 SOURCES = ["pg_abi.hip", "pg_index.hip", "pg_cell.hip", "pg_csr.hip", "pg_knn.hip", "pg_mlp.hip"]
 ARCH = "gfx950"
 
@@ -45,7 +46,7 @@ def _flags() -> list[str]:
             f"-I{INCLUDE}", "-Wno-unused-command-line-argument"]
 
 
-def build(verbose: bool | None = None) -> str:
+def build(verbose: bool | None = None, lib_path: str = LIB_PATH) -> str:
     """Compile every HIP source and link the shared library; returns its path."""
     if verbose is None:
         verbose = os.environ.get("VERBOSE") == "1"
@@ -67,16 +68,38 @@ def build(verbose: bool | None = None) -> str:
 
     with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
-    tmp = LIB_PATH + ".tmp"
+    tmp = lib_path + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, lib_path)
+    return lib_path
+
+
+def build_cpu(verbose: bool | None = None, lib_path: str = CPU_LIB_PATH) -> str:
+    """The host-core build (csrc/pg_cpu.cpp, C++/OpenMP) with the system C++
+    compiler; returns its path."""
+    if verbose is None:
+        verbose = os.environ.get("VERBOSE") == "1"
+    cxx = os.environ.get("CXX") or shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no C++ compiler found for the host-core build")
+    opt = ["-O0", "-g"] if os.environ.get("DEBUG_CUDA") == "1" else ["-O2"]
+    tmp = lib_path + ".tmp"
+    cmd = [cxx, *opt, "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off", "-Wno-unknown-pragmas",
+           f"-I{INCLUDE}", os.path.join(CSRC, "pg_cpu.cpp"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host-core build failed:\n{r.stderr}")
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpu(verbose=True))

@@ -6,7 +6,9 @@ This is the MI355X counterpart of the reference's C++ orchestration
 chunked cell construction, CSR assembly.  All device memory comes from the
 torch caching allocator; all work is ordered on
 ``torch.cuda.current_stream()`` of the points' device; the only
-device-to-host read is one 128-byte counter block that carries E.
+device-to-host read is one 192-byte counter block that carries E (and the
+row-pool counters: in the rare case that the rows did not fit, the cells run
+again with the pool size the launch reported).
 """
 from __future__ import annotations
 
@@ -19,12 +21,14 @@ import torch
 
 from ._chunking import plan_chunks
 
-NUM_STATS = 16
+NUM_STATS = 24
 STAT_NAMES = {
-    0: "fp64_rechecks", 1: "unresolved", 2: "inconsistent", 3: "overflows", 4: "clips",
+    0: "fp64_rechecks", 1: "unresolved", 2: "inconsistent", 3: "promoted", 4: "clips",
     5: "candidates", 6: "nodes", 7: "cells", 8: "asymmetric", 9: "dropped", 10: "leaves",
-    11: "skipped", 15: "edges",
+    11: "skipped", 12: "truncated", 13: "pool_short", 14: "pool_demand", 15: "edges", 16: "exact",
 }
+STAT_POOL_SHORT, STAT_POOL_DEMAND = 13, 14
+STAT_EXACT_LOG = 17  # 17..23: original ids + 1 of the first cells with an expansion-arithmetic test
 CELL_BLOCK = 64  # cells per admission block (one wavefront)
 KNN_PRECLIP = 16  # nearest neighbours clipped first when knn > 0
 
@@ -39,19 +43,22 @@ class HipLibrary:
         self.path = path
         self.lib = lib = ctypes.CDLL(path)
         i64, i32, u64p, vp = ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p
+        u64 = ctypes.c_uint64
         sig = {
             "pg_abi_version": ([], ctypes.c_int),
             "pg_last_error": ([], ctypes.c_char_p),
-            "pg_row_capacity": ([], ctypes.c_int),
             "pg_index_workspace_bytes": ([i64, u64p], ctypes.c_int),
             "pg_build_index": ([i64, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
             "pg_default_waves": ([ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
-            "pg_cell_scratch_bytes": ([i32, i64, u64p], ctypes.c_int),
-            "pg_row_buffer_elems": ([i64, u64p], ctypes.c_int),
-            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, i32, vp,
-                                  vp],
+            "pg_cell_scratch_bytes": ([i64, u64p], ctypes.c_int),
+            "pg_row_pool_elems": ([i64, u64p], ctypes.c_int),
+            "pg_compute_cells": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, u64, vp, vp, vp, vp, vp, u64, i32, vp, vp],
                                  ctypes.c_int),
             "pg_scan_tmp_bytes": ([i64, u64p], ctypes.c_int),
+            "pg_exact_side": ([i64, vp, i64, vp, vp, vp], ctypes.c_int),
+            "pg_exact_side_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_index_layout": ([i64, u64p], ctypes.c_int),
+            "pg_leaf_stats": ([i64, vp, vp, vp], ctypes.c_int),
             "pg_scan_counts": ([i64, vp, vp, vp, vp], ctypes.c_int),
             "pg_chunk_counts": ([i64, vp, i64, i64, vp, vp, vp], ctypes.c_int),
             "pg_pack_rows": ([i64, vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
@@ -68,19 +75,19 @@ class HipLibrary:
             "pg_mlp_train_scratch_elems": ([i64, u64p], ctypes.c_int),
             "pg_mlp_train_step": ([i64, vp, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp, ctypes.c_float, i32, vp],
                                   ctypes.c_int),
-            "pg_compute_cells_ex": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
-                                     ctypes.c_uint64, i32, vp, vp], ctypes.c_int),
+            "pg_compute_cells_ex": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, i32, vp, vp, vp, u64, vp, vp, vp, vp, vp,
+                                     u64, i32, vp, vp], ctypes.c_int),
             "pg_batch_workspace_bytes": ([i64, i32, u64p], ctypes.c_int),
             "pg_build_batch_index": ([i64, i32, vp, vp, vp, i32, vp, ctypes.c_uint64, vp], ctypes.c_int),
             "pg_batch_quads": ([i32, vp, i64, i64, vp], ctypes.c_int),
-            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp,
-                                        ctypes.c_uint64, i32, vp, vp], ctypes.c_int),
+            "pg_compute_cells_batch": ([i64, vp, i32, vp, i64, i64, vp, vp, i32, vp, vp, u64, vp, vp, vp, vp, vp,
+                                        u64, i32, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.pg_abi_version() != 1:
+        if lib.pg_abi_version() != 2:
             raise ImportError("libparagram_hip.so ABI version mismatch")
 
     def call(self, name: str, *args):
@@ -112,7 +119,7 @@ class Instrument:
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
                   initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
                   instrument: Instrument | None = None, clouds=None, init_h=None, extent=None,
-                  radius_model=None, preclip: int | None = None):
+                  radius_model=None, preclip: int | None = None, row_pool: int | None = None):
     """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device.
 
     ``clouds``: optional host int64 array of cloud offsets (a batch of clouds,
@@ -123,7 +130,9 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
     receives the built half-width of each cell.  ``radius_model``: a
     ``learned.RadiusModel`` whose prediction (kNN, features and network on
     this call's index) supplies ``init_h``.  ``preclip``: nearest neighbours
-    clipped first when knn > 0 (default KNN_PRECLIP, 1..16).
+    clipped first when knn > 0 (default KNN_PRECLIP, 1..16).  ``row_pool``:
+    initial row-pool elements per launch (default ``pg_row_pool_elems``);
+    a pool that proves too small grows, so only the time depends on it.
     """
     n = int(points.shape[0])
     if clouds is not None:
@@ -158,6 +167,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         if debug:
             torch.cuda.synchronize(dev)
             print("BVH Creation time: %f ms" % ((time.perf_counter() - t0) * 1e3))
+            if weights is not None:  # as laguerre_ultra.cu:259-265
+                print_leaf_statistics(lib, n, ws, sp, dev)
         t1 = time.perf_counter()
 
         waves = ctypes.c_int32(0)
@@ -167,7 +178,6 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         counts = torch.empty(n, **i32)
         status = torch.empty(n, **i32)
         row_offsets = torch.empty(n, dtype=torch.int64, device=dev)
-        rowcap = int(lib.lib.pg_row_capacity())
 
         guesses = initial_guesses
         goffs = initial_guesses_offsets
@@ -186,24 +196,32 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
                      _p(guess_d2), sp)
             goffs = None
         if initial_guesses is not None and initial_guesses_offsets is not None:
+            n_guess = guesses.numel()
+            if n_guess == 0:
+                # an empty guess list (all offsets 0) offers nothing, as in the
+                # reference (voronoi_ultra.cu:158-169, knn forced to 0); a
+                # one-entry placeholder keeps the pointer non-null
+                guesses = torch.full((1,), -1, **i32)
             # the kernel indexes guesses through the offsets: reject malformed CSR
-            ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= guesses.numel())).item())
+            ok = bool(((goffs[1:] >= goffs[:-1]).all() & (goffs[0] >= 0) & (goffs[-1] <= n_guess)).item())
             if not ok:
                 raise ValueError("initial_guesses_offsets must be non-decreasing and index into initial_guesses")
 
         total_blocks = (n + CELL_BLOCK - 1) // CELL_BLOCK
-        full_bytes = 4 * lib.u64("pg_row_buffer_elems", total_blocks * CELL_BLOCK)
+        full_bytes = 4 * lib.u64("pg_row_pool_elems", total_blocks * CELL_BLOCK)
         free_bytes, _ = torch.cuda.mem_get_info(dev)
         plan = plan_chunks(total_blocks, CELL_BLOCK, full_bytes, free_bytes)
         chunk_cells = min(plan.chunk_threads, n)
-        scratch_bytes = lib.u64("pg_cell_scratch_bytes", waves, chunk_cells)
+        scratch_bytes = lib.u64("pg_cell_scratch_bytes", chunk_cells)
         scratch = torch.empty(scratch_bytes, dtype=torch.uint8, device=dev)
-        rows = torch.empty(lib.u64("pg_row_buffer_elems", chunk_cells), **i32)
-        if debug:
-            print("Row slab: %.2f MB/chunk for %d cells, %d chunk(s) over %d points"
-                  % (rows.numel() * 4 / 2**20, chunk_cells, plan.num_chunks, n))
+        pool = [torch.empty(lib.u64("pg_row_pool_elems", chunk_cells) if row_pool is None else max(int(row_pool), 1),
+                            **i32)]
+        if debug:  # voronoi_ultra.cu:274-281 (here the row pool and the cell scratch)
+            print("ConvexCell Memory: %.2f MB/chunk for %d threads, %d chunk(s) over %d points"
+                  % ((pool[0].numel() * 4 + scratch_bytes) / 2**20, chunk_cells, plan.num_chunks, n))
 
         def cells(begin, end):
+            rows = pool[0]
             ev = None
             if instrument is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -211,20 +229,40 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             cyc = _p(instrument.cycles if instrument is not None else None)
             if clouds is None:
                 lib.call("pg_compute_cells_ex", n, _p(ws), begin, end, _p(guesses), _p(goffs), 0, guess_stride,
-                         _p(guess_d2), seed_window, _p(init_h), _p(extent), _p(rows), _p(row_offsets), _p(counts), _p(status),
-                         _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
+                         _p(guess_d2), seed_window, _p(init_h), _p(extent), _p(rows), rows.numel(), _p(row_offsets),
+                         _p(counts), _p(status), _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
             else:
                 lib.call("pg_compute_cells_batch", n, _p(ws), n_clouds, c_off, begin, end, _p(guesses), _p(goffs),
-                         guess_stride, _p(guess_d2), _p(rows), _p(row_offsets), _p(counts), _p(status), _p(stats), _p(scratch),
-                         scratch_bytes, waves, cyc, sp)
+                         guess_stride, _p(guess_d2), _p(rows), rows.numel(), _p(row_offsets), _p(counts), _p(status),
+                         _p(stats), _p(scratch), scratch_bytes, waves, cyc, sp)
             if ev is not None:
                 ev[1].record(stream)
                 instrument.cell_events.append(ev)
 
+        def grow(need):
+            """The rows of a launch did not fit the row pool: a pool of the size
+            the launch reported, for running it again."""
+            pool[0] = None
+            pool[0] = torch.empty(need + need // 8 + 65536, **i32)
+            if debug:
+                print("Row pool grown to %.2f MB (a launch needed %d entries)" % (pool[0].numel() * 4 / 2**20, need))
+
         scan_tmp = torch.empty(max(lib.u64("pg_scan_tmp_bytes", n + 1), 4), dtype=torch.uint8, device=dev)
+        offsets = torch.empty(n + 1, **i32)
+        stride = 0
         if plan.num_chunks == 1:
-            cells(0, n)
-            src, src_off = rows, row_offsets
+            while True:
+                snap = stats.clone()
+                cells(0, n)
+                src, src_off = pool[0], row_offsets
+                lib.call("pg_scan_counts", n, _p(counts), _p(offsets), _p(scan_tmp), sp)
+                lib.call("pg_check_symmetry", n, _p(ws), _p(counts), _p(src), _p(src_off), stride, _p(offsets),
+                         _p(stats), sp)
+                st = stats.cpu()  # the single device-to-host read: E, the symmetry and pool counters
+                if int(st[STAT_POOL_SHORT]) == 0:
+                    break
+                stats.copy_(snap)
+                grow(int(st[STAT_POOL_DEMAND]))
         else:
             src_off = torch.empty(n, dtype=torch.int64, device=dev)
             parts, base = [], 0
@@ -233,26 +271,32 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
                 if begin >= n:
                     break
                 end = min(n, begin + chunk_cells)
-                cells(begin, end)
                 m = end - begin
-                cc = torch.empty(m, **i32)
-                lib.call("pg_chunk_counts", n, _p(ws), begin, end, _p(counts), _p(cc), sp)
-                loc = torch.empty(m + 1, **i32)
-                lib.call("pg_scan_counts", m, _p(cc), _p(loc), _p(scan_tmp), sp)
-                total = int(loc[m].item())  # chunked mode: one read per chunk
+                while True:
+                    snap = stats.clone()
+                    cells(begin, end)
+                    cc = torch.empty(m, **i32)
+                    lib.call("pg_chunk_counts", n, _p(ws), begin, end, _p(counts), _p(cc), sp)
+                    loc = torch.empty(m + 1, **i32)
+                    lib.call("pg_scan_counts", m, _p(cc), _p(loc), _p(scan_tmp), sp)
+                    # chunked mode: one read per chunk (its row total and pool counters)
+                    vals = torch.cat([loc[m:m + 1].to(torch.int64), stats[STAT_POOL_SHORT:STAT_POOL_DEMAND + 1]]).cpu()
+                    if int(vals[1]) == 0:
+                        break
+                    stats.copy_(snap)
+                    grow(int(vals[2]))
+                total = int(vals[0])
                 packed = torch.empty(max(total, 1), **i32)
-                lib.call("pg_pack_rows", n, _p(ws), begin, end, _p(counts), _p(loc), _p(rows), _p(row_offsets),
+                lib.call("pg_pack_rows", n, _p(ws), begin, end, _p(counts), _p(loc), _p(pool[0]), _p(row_offsets),
                          _p(packed), sp)
                 src_off[begin:end] = loc[:m].to(torch.int64) + base
                 parts.append(packed[:total])
                 base += total
             src = torch.cat(parts) if base > 0 else torch.empty(1, **i32)
-        stride = 0
-
-        offsets = torch.empty(n + 1, **i32)
-        lib.call("pg_scan_counts", n, _p(counts), _p(offsets), _p(scan_tmp), sp)
-        lib.call("pg_check_symmetry", n, _p(ws), _p(counts), _p(src), _p(src_off), stride, _p(offsets), _p(stats), sp)
-        st = stats.cpu()  # the single device-to-host read: E and the symmetry counters
+            lib.call("pg_scan_counts", n, _p(counts), _p(offsets), _p(scan_tmp), sp)
+            lib.call("pg_check_symmetry", n, _p(ws), _p(counts), _p(src), _p(src_off), stride, _p(offsets),
+                     _p(stats), sp)
+            st = stats.cpu()  # E and the symmetry counters
         e_total = int(st[15])
         adjacency = torch.empty(max(e_total, 1), **i32)
         lib.call("pg_emit_csr", n, _p(ws), _p(counts), _p(offsets), _p(src), _p(src_off), stride,
@@ -266,8 +310,30 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             print("%s Construction time: %f ms" % (name, (time.perf_counter() - t1) * 1e3))
         if instrument is not None:
             instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
+            instrument.stats["exact_cells"] = [int(v) - 1 for v in st[STAT_EXACT_LOG:NUM_STATS].tolist() if v > 0]
             instrument.plan = plan
         return adjacency, offsets, status
+
+
+def leaf_statistics(lib: HipLibrary, n: int, ws: torch.Tensor, sp, dev):
+    """(leaves, min, max, total points) of the index's leaves."""
+    out = torch.empty(4, dtype=torch.int64, device=dev)
+    lib.call("pg_leaf_stats", n, _p(ws), _p(out), sp)
+    return tuple(int(v) for v in out.cpu().tolist())
+
+
+def print_leaf_statistics(lib: HipLibrary, n: int, ws: torch.Tensor, sp, dev):
+    """The debug print of pwr_bvh.cu:444-449."""
+    leaves, lo, hi, total = leaf_statistics(lib, n, ws, sp, dev)
+    if leaves == 0:
+        print("BVH Leaf Statistics: No leaves found")
+        return
+    print("BVH Leaf Statistics:")
+    print("  Total leaves: %d" % leaves)
+    print("  Min points per leaf: %d" % lo)
+    print("  Max points per leaf: %d" % hi)
+    print("  Mean points per leaf: %.2f" % (total / leaves))
+    print("  Total points in leaves: %d" % total)
 
 
 def knn(lib: HipLibrary, points: torch.Tensor, k: int = 8, leaf_size: int = -1):

@@ -3,6 +3,7 @@
 // work is in pg_index.hip, pg_cell.hip and pg_csr.hip.
 #include "../../include/paragram_hip.h"
 #include "pg_common.hpp"
+#include "pg_exact.hpp"
 #include "pg_kernels.hpp"
 
 #include <stdio.h>
@@ -39,8 +40,6 @@ int pg_abi_version(void) { return PG_ABI_VERSION; }
 
 const char *pg_last_error(void) { return g_last_error.c_str(); }
 
-int pg_row_capacity(void) { return pg::kRowCap; }
-
 int pg_index_workspace_bytes(int64_t n, uint64_t *bytes) {
     if (int r = check_n(n)) return r;
     if (!bytes) return fail(1, "bytes is null");
@@ -58,6 +57,26 @@ int pg_build_index(int64_t n, const float *points, const float *weights, int32_t
     if (leaf < 1) leaf = 1;
     if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
     return check_hip(pg::build_index(n, points, weights, leaf, workspace, as_stream(stream)), "build_index");
+}
+
+int pg_index_layout(int64_t n, uint64_t *offsets) {
+    if (int r = check_n(n)) return r;
+    if (!offsets) return fail(1, "offsets is null");
+    const pg::IndexLayout L = pg::make_index_layout(n);
+    offsets[0] = L.header;
+    offsets[1] = L.points;
+    offsets[2] = L.perm;
+    offsets[3] = L.inv;
+    offsets[4] = L.nodes;
+    offsets[5] = L.total;
+    return 0;
+}
+
+int pg_leaf_stats(int64_t n, const void *workspace, uint64_t *out, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n == 0) return 0;
+    if (!workspace || !out) return fail(1, "workspace and out must be non-null");
+    return check_hip(pg::leaf_stats(n, workspace, (unsigned long long *)out, as_stream(stream)), "leaf_stats");
 }
 
 int pg_batch_workspace_bytes(int64_t n, int32_t n_clouds, uint64_t *bytes) {
@@ -190,37 +209,35 @@ int pg_default_waves(int32_t *waves) {
     return 0;
 }
 
-int pg_cell_scratch_bytes(int32_t waves, int64_t max_cells, uint64_t *bytes) {
-    if (waves < 1) return fail(1, "waves must be positive");
+int pg_cell_scratch_bytes(int64_t max_cells, uint64_t *bytes) {
     if (max_cells < 0) return fail(1, "max_cells must be non-negative");
+    if (max_cells > pg::kMaxPoints) return fail(1, "too many cells for one launch (limit 67108863)");
     if (!bytes) return fail(1, "bytes is null");
-    *bytes = pg::cell_scratch_bytes((waves + 3) / 4 * 4, pg::kTier1Waves, pg::tier1_capacity(max_cells),
-                                    pg::tier2_capacity(max_cells), max_cells);
+    *bytes = pg::cell_scratch_bytes(max_cells);
     return 0;
 }
 
-int pg_row_buffer_elems(int64_t cells, uint64_t *elems) {
+int pg_row_pool_elems(int64_t cells, uint64_t *elems) {
     if (cells < 0 || !elems) return fail(1, "invalid argument");
-    *elems = (uint64_t)pg::row_buffer_elems(cells);
+    *elems = (uint64_t)pg::row_pool_elems(cells);
     return 0;
 }
 
 int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                      const int32_t *guess_offsets, int32_t guesses_sorted, int32_t seed_window, int32_t *rows,
-                     int64_t *row_offsets,
-                     int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                     int32_t waves, uint32_t *cell_cycles, void *stream) {
+                     uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
+                     void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream) {
     return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, 0, nullptr,
-                               seed_window, nullptr, nullptr, rows, row_offsets, counts, status, stats, scratch, scratch_bytes,
-                               waves, cell_cycles, stream);
+                               seed_window, nullptr, nullptr, rows, rows_elems, row_offsets, counts, status, stats,
+                               scratch, scratch_bytes, waves, cell_cycles, stream);
 }
 
 int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
                         const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
                         const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
-                        int32_t *rows, int64_t *row_offsets,
-                        int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                        int32_t waves, uint32_t *cell_cycles, void *stream) {
+                        int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status,
+                        uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves,
+                        uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
     if (begin == end) return 0;
@@ -232,10 +249,7 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
     if (waves < 1) return fail(1, "waves must be positive");
     const int w0 = (waves + 3) / 4 * 4;
-    const int64_t big_cap = pg::tier1_capacity(end - begin);
-    const int64_t huge_cap = pg::tier2_capacity(end - begin);
-    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap, end - begin))
-        return fail(1, "cell scratch too small");
+    if (scratch_bytes < pg::cell_scratch_bytes(end - begin)) return fail(1, "cell scratch too small");
     pg::CellArgs a;
     a.guess_stride = guess_stride;
     a.guess_d2 = guess_stride > 0 ? guess_d2 : nullptr;
@@ -248,7 +262,7 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     a.guesses_sorted = guesses_sorted;
     a.seed_window = seed_window < 0 ? 0 : (seed_window > 16 ? 16 : seed_window);
     a.rows = rows;
-    a.row_stride = pg::kRowCap;
+    a.rows_elems = (int64_t)rows_elems;
     a.row_offsets = row_offsets;
     a.counts = counts;
     a.status = status;
@@ -257,8 +271,6 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     a.scratch_bytes = scratch_bytes;
     a.waves = w0;
     a.waves1 = w0 / 2 > pg::kTier1Waves ? w0 / 2 : pg::kTier1Waves;  // 8 per CU: what tier 1 keeps resident
-    a.big_cap = big_cap;
-    a.huge_cap = huge_cap;
     a.cycles = cell_cycles;
     a.init_h = init_half_width;
     a.extent = extent;
@@ -267,9 +279,9 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
 
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
                            int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t guess_stride, const float *guess_d2, int32_t *rows, int64_t *row_offsets,
-                           int32_t *counts, int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
-                           int32_t waves, uint32_t *cell_cycles, void *stream) {
+                           int32_t guess_stride, const float *guess_d2, int32_t *rows, uint64_t rows_elems,
+                           int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
+                           uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream) {
     if (int r = check_n(n)) return r;
     if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
     if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
@@ -278,10 +290,7 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
         return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
     if (waves < 1) return fail(1, "waves must be positive");
     const int w0 = (waves + 3) / 4 * 4;
-    const int64_t big_cap = pg::tier1_capacity(end - begin);
-    const int64_t huge_cap = pg::tier2_capacity(end - begin);
-    if (scratch_bytes < pg::cell_scratch_bytes(w0, pg::kTier1Waves, big_cap, huge_cap, end - begin))
-        return fail(1, "cell scratch too small");
+    if (scratch_bytes < pg::cell_scratch_bytes(end - begin)) return fail(1, "cell scratch too small");
     const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
     char *ws = (char *)workspace;
     std::vector<int64_t> quads((size_t)n_clouds + 1);
@@ -307,7 +316,7 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.guesses_sorted = 0;
     a.seed_window = 0;
     a.rows = rows;
-    a.row_stride = pg::kRowCap;
+    a.rows_elems = (int64_t)rows_elems;
     a.row_offsets = row_offsets;
     a.counts = counts;
     a.status = status;
@@ -316,14 +325,37 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.scratch_bytes = scratch_bytes;
     a.waves = w0;
     a.waves1 = w0 / 2 > pg::kTier1Waves ? w0 / 2 : pg::kTier1Waves;  // 8 per CU: what tier 1 keeps resident
-    a.big_cap = big_cap;
-    a.huge_cap = huge_cap;
     a.cycles = cell_cycles;
     a.clouds = (const pg::IndexHeader *)(ws + B.clouds);
     a.cloud_cells = (const int64_t *)(ws + B.offsets);
     a.quads = (const int64_t *)(ws + B.quads);
     a.n_clouds = n_clouds;
     return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells_batch");
+}
+
+int pg_exact_side(int64_t n, const void *workspace, int64_t m, const int32_t *queries, int32_t *out, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (m < 0) return fail(1, "number of queries must be non-negative");
+    if (m == 0) return 0;
+    if (n == 0 || !workspace || !queries || !out) return fail(1, "workspace, queries and out must be non-null");
+    return check_hip(pg::exact_side(n, workspace, m, queries, out, as_stream(stream)), "exact_side");
+}
+
+int pg_exact_side_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
+                       const float *box_hi, int32_t *out) {
+    if (!seed || !planes || !box_codes || !box_lo || !box_hi || !out) return fail(1, "invalid argument");
+    pg::xp::PlaneIn pl[4];
+    for (int e = 0; e < 4; ++e) {
+        if (box_codes[e] < -1 || box_codes[e] > 5) return fail(1, "box codes must be -1 (point) or 0..5");
+        pl[e] = pg::xp::PlaneIn{planes[4 * e], planes[4 * e + 1], planes[4 * e + 2], planes[4 * e + 3], box_codes[e]};
+    }
+    int sD = 0, sP = 0;
+    if (!pg::xp::exact_signs(pl, seed, box_lo, box_hi, sD, sP)) {
+        out[0] = 2, out[1] = 0, out[2] = 0;
+        return 0;
+    }
+    out[0] = sD * sP, out[1] = sD, out[2] = sP;
+    return 0;
 }
 
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes) {

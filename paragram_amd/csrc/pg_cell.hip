@@ -22,9 +22,10 @@
 // positions are kept with a rigorous error radius derived from those
 // bounds; a removal test is decided in FP32 only when its value exceeds its
 // error bound, and otherwise recomputed in FP64 from the FP32 inputs with the
-// bound 1e-13 x permanent (gamma_24 < 2.7e-15).  Only a test that FP64 cannot
-// decide either is counted as unresolved.
+// bound 1e-13 x permanent (gamma_24 < 2.7e-15); a test that FP64 cannot decide
+// either is decided exactly by expansion arithmetic (pg_exact.hpp).
 #include "pg_common.hpp"
+#include "pg_exact.hpp"
 #include "pg_kernels.hpp"
 
 #include <float.h>
@@ -53,8 +54,8 @@ struct CellParams {
     int guess_stride;             // > 0: guesses are Morton-ordered rows of this many sorted ids
     const float *guess_d2;        // with guess_stride (may be null): squared distance of each row's last id
     int seed_window;              // Morton neighbours offered first (0: none)
-    int32_t *rows;                // tier-0 slab (row_stride per cell) then tier-1 rows
-    int row_stride;
+    int32_t *rows;                // row pool (every tier bump-allocates its rows)
+    uint64_t rows_elems;          // pool capacity
     int64_t *row_offsets;         // [n] sorted cell -> row start in `rows`
     int32_t *counts;
     int32_t *status;
@@ -62,10 +63,10 @@ struct CellParams {
     uint32_t *counters;           // [0] tier-0 work, [1] tier-1 queue length, [2] tier-1 work,
                                   // [3] tier-2 queue length, [4] tier-2 work,
                                   // [5] cells flagged for FP64 tier 0, [6] FP64 tier-0 windows
-    uint32_t *big_list;           // tier-1 queue of sorted cell ids
-    uint32_t big_cap;
-    uint32_t *huge_list;          // tier-2 queue of sorted cell ids
-    uint32_t huge_cap;
+    unsigned long long *pool_top; // row-pool allocation counter (in the scratch, zeroed per launch)
+    uint32_t *big_list;           // tier-1 queue of sorted cell ids (room for every cell)
+    uint32_t *huge_list;          // tier-2 queue of sorted cell ids (room for every cell)
+    uint32_t queue_cap;           // end - begin
     uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
     uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
     // batch of clouds (null clouds: one cloud described by hdr)
@@ -104,17 +105,62 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 namespace pg {
 
-__global__ void cell_prep_kernel(uint32_t *counters) {
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+// One thread per query (seed, three vertex planes, query plane): ids >= 0 are
+// original point ids, ids < 0 box-plane codes -1 - (2 axis + side).  Writes
+// (sign(P) sign(D), sign(D), sign(P)), or (2, 0, 0) when an expansion
+// exceeded its capacity and (3, 0, 0) for an id out of range.
+__global__ void exact_side_kernel(int64_t m, int64_t n, const IndexHeader *hdr, const float4 *pts,
+                                  const uint32_t *inv, const int32_t *queries, int32_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t *q = queries + 5 * i;
+    bool ok = q[0] >= 0 && q[0] < n;
+    xp::PlaneIn pl[4];
+    for (int e = 0; e < 4; ++e) {
+        const int32_t id = q[1 + e];
+        ok = ok && id >= -6 && id < n;
+        if (ok && id >= 0) {
+            const float4 p = pts[inv[id]];
+            pl[e] = xp::PlaneIn{p.x, p.y, p.z, p.w, -1};
+        } else {
+            pl[e] = xp::PlaneIn{0.f, 0.f, 0.f, 0.f, ok ? -1 - id : 0};
+        }
+    }
+    int32_t *o = out + 3 * i;
+    if (!ok) {
+        o[0] = 3, o[1] = 0, o[2] = 0;
+        return;
+    }
+    const float4 sp = pts[inv[q[0]]];
+    const float s4[4] = {sp.x, sp.y, sp.z, sp.w};
+    int sD = 0, sP = 0;
+    if (!xp::exact_signs(pl, s4, hdr->box_lo, hdr->box_hi, sD, sP)) {
+        o[0] = 2, o[1] = 0, o[2] = 0;
+        return;
+    }
+    o[0] = sD * sP, o[1] = sD, o[2] = sP;
 }
 
-// Every tier keeps its cells in LDS: the scratch holds the work counters
-// and the queues (FP64 tier 0: every cell of the range; tiers 1 and 2).
-uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells) {
-    (void)waves;
-    (void)waves1;
-    return 256 + 4 * (uint64_t)big_cap + 4 * (uint64_t)huge_cap + 4 * (uint64_t)cells;
+hipError_t exact_side(int64_t n, const void *index_ws, int64_t m, const int32_t *queries, int32_t *out,
+                      hipStream_t stream) {
+    if (m <= 0) return hipSuccess;
+    const IndexLayout L = make_index_layout(n);
+    const char *ws = (const char *)index_ws;
+    hipLaunchKernelGGL(exact_side_kernel, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, stream, m, n,
+                       (const IndexHeader *)(ws + L.header), (const float4 *)(ws + L.points),
+                       (const uint32_t *)(ws + L.inv), queries, out);
+    return hipGetLastError();
 }
+
+__global__ void cell_prep_kernel(uint32_t *counters) {
+    if (threadIdx.x < 64) counters[threadIdx.x] = 0;  // work counters and the pool counter
+}
+
+// Every tier keeps its cells in LDS: the scratch holds the work counters,
+// the row-pool counter and three per-cell lists (the FP64 tier-0 flags and
+// the tier-1 and tier-2 queues, each with room for every cell, so no queue
+// can fill).
+uint64_t cell_scratch_bytes(int64_t cells) { return 256 + 12 * (uint64_t)cells; }
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     const IndexLayout L = make_index_layout(a.n);
@@ -138,7 +184,7 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.guess_d2 = a.guess_d2;
     prm.seed_window = a.seed_window;
     prm.rows = a.rows;
-    prm.row_stride = a.row_stride;
+    prm.rows_elems = (uint64_t)a.rows_elems;
     prm.row_offsets = a.row_offsets;
     prm.counts = a.counts;
     prm.status = a.status;
@@ -150,17 +196,16 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.n_clouds = a.n_clouds;
     prm.init_h = a.init_h;
     prm.extent = a.extent;
-    char *sc = (char *)a.scratch;
-    prm.counters = (uint32_t *)sc;
-    prm.big_list = (uint32_t *)(sc + 256);
-    prm.big_cap = (uint32_t)a.big_cap;
-    prm.huge_list = (uint32_t *)(sc + 256 + 4 * a.big_cap);
-    prm.huge_cap = (uint32_t)a.huge_cap;
-    prm.exact_list = (uint32_t *)(sc + 256 + 4 * a.big_cap + 4 * a.huge_cap);
     const int64_t count = a.end - a.begin;
     if (count <= 0) return hipSuccess;
-    if (cell_scratch_bytes(a.waves, a.waves1, a.big_cap, a.huge_cap, count) > a.scratch_bytes)
-        return hipErrorInvalidValue;
+    if (cell_scratch_bytes(count) > a.scratch_bytes || count > (int64_t)0xFFFFFFFF) return hipErrorInvalidValue;
+    char *sc = (char *)a.scratch;
+    prm.counters = (uint32_t *)sc;
+    prm.pool_top = (unsigned long long *)(sc + 128);
+    prm.queue_cap = (uint32_t)count;
+    prm.big_list = (uint32_t *)(sc + 256);
+    prm.huge_list = (uint32_t *)(sc + 256 + 4 * count);
+    prm.exact_list = (uint32_t *)(sc + 256 + 8 * count);
     const int64_t cells_per_block = kT0Threads / kT0Lanes;
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;

@@ -7,29 +7,18 @@
 namespace pg {
 
 // ------------------------------------------------------------------ limits
-// Row strides of the three capacity tiers of the cell kernel (the tiers and
-// their plane / vertex capacities are defined in pg_group_cell.hpp).  A cell
-// of F faces has 2F-4 vertices.
-constexpr int kRowCap = 48;            // tier-0 row stride (= the tier-0 plane capacity)
-constexpr int kTier1Row = 512;         // tier-1 row stride (255 planes, plus duplicates)
+// Rows of every capacity tier (the tiers and their plane / vertex capacities
+// are defined in pg_group_cell.hpp) are bump-allocated from one row pool, so
+// a row takes exactly its length.  A cell of F faces has 2F-4 vertices.
 constexpr int kTier1Waves = 1024;      // least persistent waves of the tier-1 launch (8 per CU where more)
-constexpr int kTier2Row = 2048;        // tier-2 row stride (1023 planes, plus duplicates)
 constexpr int kTier2Waves = 256;       // persistent waves of the tier-2 launch
+constexpr int kPoolPerCell = 32;       // default row-pool entries per cell (a Poisson cell has 15.5 faces)
+constexpr int64_t kPoolSlack = 65536;  // plus this many entries per launch
 
-// Tier-1 queue capacity for a launch over m cells.
-__host__ __device__ inline int64_t tier1_capacity(int64_t m) {
-    const int64_t c = m / 32 + 256;
-    return c < m ? c : m;
-}
-// Tier-2 queue capacity for a launch over m cells.
-__host__ __device__ inline int64_t tier2_capacity(int64_t m) {
-    const int64_t c = m / 1024 + 64;
-    return c < m ? c : m;
-}
-// Elements of the row buffer of a launch over m cells.
-__host__ __device__ inline int64_t row_buffer_elems(int64_t m) {
-    return m * kRowCap + tier1_capacity(m) * kTier1Row + tier2_capacity(m) * kTier2Row;
-}
+// Default row-pool elements of a launch over m cells.  A launch whose rows
+// do not fit reports it (kStatPoolShort, with the size needed in
+// kStatPoolDemand) and the host runs it again with a pool of that size.
+__host__ __device__ inline int64_t row_pool_elems(int64_t m) { return m * kPoolPerCell + kPoolSlack; }
 constexpr int kStackCap = 64;          // LBVH depth is < 56 for N < 2^26
 constexpr int kMaxLeaf = 32;
 constexpr int kDefaultLeaf = 32;      // points per LBVH leaf (two 16-lane batches); measured fastest
@@ -49,9 +38,9 @@ enum Status : int32_t {
 // Device-side counters, accumulated with one atomic per wave.
 enum StatSlot : int {
     kStatFp64Rechecks = 0,   // FP32 filter undecided, recomputed in FP64
-    kStatUnresolved = 1,     // FP64 also undecided (decided by the FP64 sign)
+    kStatUnresolved = 1,     // FP64 and expansion arithmetic both undecided (FP64 sign taken)
     kStatInconsistent = 2,   // orientation or boundary inconsistency
-    kStatOverflow = 3,       // plane/vertex capacity exceeded
+    kStatPromoted = 3,       // cells passed on to a larger-capacity tier
     kStatClips = 4,          // successful clips
     kStatCandidates = 5,     // candidates that reached the vertex tests
     kStatNodes = 6,          // internal nodes visited
@@ -60,8 +49,16 @@ enum StatSlot : int {
     kStatDropped = 9,        // row entries pointing at empty cells
     kStatLeaves = 10,        // leaves visited
     kStatSkipped = 11,       // candidates rejected by the support test
-    kNumStats = 16,
+    kStatTruncated = 12,     // cells beyond the largest tier: status 1, empty row
+    kStatPoolShort = 13,     // rows that did not fit the row pool (the launch must run again)
+    kStatPoolDemand = 14,    // (max, not sum) row-pool elements a launch that ran short needed
+    kStatEdges = 15,         // edge total E (written by the symmetry check)
+    kStatExact = 16,         // removal tests decided by expansion arithmetic (FP64 undecided)
+    kStatExactLog = 17,      // 17..23: original ids of the first cells with such a test, plus 1
+    kNumStats = 24,
 };
+constexpr int kExactLogSlots = 7;
+constexpr int kNumCtr = 12;  // statistics slots 0..11 are first summed per group in LDS
 
 // ------------------------------------------------------------------ LBVH
 // Internal node of the Karras hierarchy in traversal form: the boxes of both
@@ -115,6 +112,7 @@ struct IndexLayout {
     uint64_t scan_tmp;   // uint32 block sums of the histogram scan
     uint64_t dup_head;   // uint32 [n]  first sorted index of the point's duplicate group
     uint64_t dup_next;   // uint32 [n]  next member of the group (0xFFFFFFFF at the end)
+    uint64_t dup_table;  // uint32 [2n] hash table of the points in long equal-key runs (build only)
     uint64_t total;
     int sort_blocks;
     int reduce_blocks;
@@ -160,6 +158,7 @@ inline IndexLayout make_index_layout(int64_t n) {
     L.scan_tmp = take(4 * ((256ull * L.sort_blocks + 4095) / 4096 + 2));
     L.dup_head = take(4 * nn);
     L.dup_next = take(4 * nn);
+    L.dup_table = take(8 * nn);
     L.total = o;
     return L;
 }

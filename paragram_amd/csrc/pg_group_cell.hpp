@@ -43,7 +43,8 @@ struct alignas(16) GroupLDS {
     uint32_t vtri[V];
     uint32_t tmp[V];
     uint32_t pm[P <= 64 ? (P + 1) / 2 : 1];
-    uint32_t ctr[12];  // diagnostics, in the order of the statistics slots
+    uint32_t ctr[kNumCtr];  // diagnostics, in the order of the statistics slots
+    uint32_t exact_events;  // removal tests of the current cell decided by expansion arithmetic
 };
 
 // Candidates of one batch, staged for the group: (a, H) and (pH, owner).
@@ -176,10 +177,35 @@ __device__ __forceinline__ void fp64_vertex(const float2 *plx, const float4 *pts
     }
 }
 
+// The removal test decided exactly (expansion arithmetic, pg_exact.hpp) for
+// the vertex with plane triple t against the plane of point q_owner: bit 0
+// beyond (sign(P) sign(D) > 0), bit 1 capacity exceeded (no decision).  Its
+// own call site keeps the expansions' private arrays off the common path.
+template <int kB>
+__device__ __noinline__ int exact_removal(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+                                          float sx, float sy, float sz, float sw, int q_owner) {
+    xp::PlaneIn pl[4];
+    for (int e = 0; e < 4; ++e) {
+        const int o = e < 3 ? __float_as_int(plx[tri_get<kB>(t, e)].y) : q_owner;
+        if (o >= 0) {
+            const float4 p = pts[o];
+            pl[e] = xp::PlaneIn{p.x, p.y, p.z, p.w, -1};
+        } else {
+            pl[e] = xp::PlaneIn{0.f, 0.f, 0.f, 0.f, -1 - o};
+        }
+    }
+    const float s4[4] = {sx, sy, sz, sw};
+    int sD = 0, sP = 0;
+    if (!xp::exact_signs(pl, s4, hdr->box_lo, hdr->box_hi, sD, sP)) return 2;
+    return sD * sP > 0 ? 1 : 0;
+}
+
 // FP64 removal decision for the vertex with plane triple t against the
-// plane of point q_owner: P = a_q . N - H_q D > 0.  Returns bit 0: beyond,
-// bit 1: undecided even in FP64 (sign of the FP64 value taken), bit 2: the
-// vertex orientation D is certainly negative.
+// plane of point q_owner: beyond iff sign(P) sign(D) > 0 with
+// P = a_q . N - H_q D (D > 0 for every vertex a cell keeps).  Returns bit 0:
+// beyond, bit 1: undecided even by expansion arithmetic (capacity; the FP64
+// signs taken), bit 2: the vertex orientation D is certainly negative,
+// bit 3: decided by expansion arithmetic (the FP64 filter could not).
 template <int kB>
 __device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
                                          float sx_, float sy_, float sz_, float sw_, int q_owner) {
@@ -191,8 +217,9 @@ __device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, c
     const double Pv = a[0] * N[0] + a[1] * N[1] + a[2] * N[2] - H * D;
     const double pP = pa[0] * pN[0] + pa[1] * pN[1] + pa[2] * pN[2] + pH * pD;
     int flags = 0;
-    if (fabs(D) > kG64 * pD && pD > kTiny64 && D < 0.0) flags |= 4;
-    if (fabs(Pv) > kG64 * pP && pP > kTiny64) return flags | (Pv > 0.0 ? 1 : 0);
+    const bool d_sure = fabs(D) > kG64 * pD && pD > kTiny64;
+    if (d_sure && D < 0.0) flags |= 4;
+    if (d_sure && fabs(Pv) > kG64 * pP && pP > kTiny64) return flags | ((Pv > 0.0) == (D > 0.0) ? 1 : 0);
     // A point identical (position and weight) to the owner of one of the
     // vertex's planes gives the same plane: P = 0 exactly, not beyond.
     const float4 q = pts[q_owner];
@@ -202,7 +229,9 @@ __device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, c
         const float4 r = pts[o];
         if (r.x == q.x && r.y == q.y && r.z == q.z && r.w == q.w) return flags;
     }
-    return flags | 2 | (Pv > 0.0 ? 1 : 0);
+    const int x = exact_removal<kB>(plx, pts, hdr, t, sx_, sy_, sz_, sw_, q_owner);
+    if (!(x & 2)) return flags | 8 | (x & 1);
+    return flags | 2 | ((Pv > 0.0) == (D >= 0.0) ? 1 : 0);
 }
 
 // Position and error radius of a vertex whose FP32 orientation is not
@@ -399,6 +428,7 @@ struct Group {
         const int r = fp64_recheck<kB>(S->plx, pts, hdr, S->vtri[v], sx, sy, sz, sw, q_owner);
         if (r & 2) count(kStatUnresolved);
         if (r & 4) count(kStatInconsistent);
+        if (r & 8) atomicAdd(&S->exact_events, 1u);
         return (r & 1) != 0;
     }
 
@@ -1114,67 +1144,84 @@ struct Group {
 
     // ---------------------------------------------------------- output
     // Row = sorted original ids of the owners of referenced planes
-    // (voronoi_ultra.cu:125-138).  Returns false when the row does not fit.
-    __device__ __forceinline__ bool emit(const CellParams &prm, uint32_t orig, int32_t *row_out, int row_cap) {
-        int F = 0;
-        if (status != kEmptyCell) {
-            uint64_t used[kPW];
-            used_planes(used);
+    // (voronoi_ultra.cu:125-138).  stage_row() returns the row length F
+    // (group-uniform; the ids are staged unsorted in tmp[0, F)), the caller
+    // takes F entries of the row pool, and write_row() writes the sorted row
+    // there.  With duplicate points a row also holds every exact duplicate of
+    // an owner (a second copy of a plane removes no vertex, so at most one
+    // plane per group of duplicates is referenced); lane 0 builds such rows.
+    __device__ __forceinline__ int stage_row(const CellParams &prm) const {
+        if (status == kEmptyCell) return 0;
+        uint64_t used[kPW];
+        used_planes(used);
+        if (hdr->has_duplicates != 0u) {
+            int F = 0;
+            if (gl == 0) {
+                for (int p = 0; p < np; ++p) {
+                    if (!mask_bit(used, p)) continue;
+                    const int owner = __float_as_int(S->plx[p].y);
+                    if (owner < 0) continue;
+                    for (uint32_t m = prm.dup_head[owner]; m != 0xFFFFFFFFu; m = prm.dup_next[m]) F++;
+                }
+            }
+            return gbi(F, 0);
+        }
+        int32_t ids[kPRows];
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < kPRows; ++k) {
+            const int p = gl + G * k;
+            ids[k] = -1;
+            if (p < np && mask_bit(used, p)) {
+                const int owner = __float_as_int(S->plx[p].y);
+                if (owner >= 0) {
+                    ids[k] = (int32_t)perm[owner];
+                    mine++;
+                }
+            }
+        }
+        int F;
+        int at = gexcl(mine, F);
+#pragma unroll
+        for (int k = 0; k < kPRows; ++k)
+            if (ids[k] >= 0) S->tmp[at++] = (uint32_t)ids[k];
+        return F;
+    }
+
+    // Writes the staged row (sorted) to row_out and the cell's count, status
+    // and extent.
+    __device__ __forceinline__ void write_row(const CellParams &prm, uint32_t orig, int32_t *row_out, int F) const {
+        if (F > 0) {
             if (hdr->has_duplicates != 0u) {
-                // every exact duplicate of an owner carries the same face
-                bool full = false;
+                // insertion into the sorted row, dropping repeats
+                int len = 0;
+                uint64_t used[kPW];
+                used_planes(used);
                 if (gl == 0) {
                     for (int p = 0; p < np; ++p) {
                         if (!mask_bit(used, p)) continue;
                         const int owner = __float_as_int(S->plx[p].y);
                         if (owner < 0) continue;
-                        uint32_t m = prm.dup_head[owner];
-                        while (m != 0xFFFFFFFFu) {
+                        for (uint32_t m = prm.dup_head[owner]; m != 0xFFFFFFFFu; m = prm.dup_next[m]) {
                             const int32_t id = (int32_t)perm[m];
-                            int k = F;
+                            int k = len;
                             while (k > 0 && row_out[k - 1] > id) k--;
-                            if (!(k > 0 && row_out[k - 1] == id)) {
-                                if (F < row_cap) {
-                                    for (int t = F; t > k; --t) row_out[t] = row_out[t - 1];
-                                    row_out[k] = id;
-                                    F++;
-                                } else {
-                                    full = true;
-                                }
-                            }
-                            m = prm.dup_next[m];
+                            if (k > 0 && row_out[k - 1] == id) continue;
+                            if (len >= F) break;  // (F bounds the distinct ids)
+                            for (int t = len; t > k; --t) row_out[t] = row_out[t - 1];
+                            row_out[k] = id;
+                            len++;
                         }
                     }
                 }
-                if (gany(full)) return false;
-                F = gbi(F, 0);
+                F = gbi(len, 0);
             } else {
-                int32_t ids[kPRows];
-                int mine = 0;
-#pragma unroll
-                for (int k = 0; k < kPRows; ++k) {
-                    const int p = gl + G * k;
-                    ids[k] = -1;
-                    if (p < np && mask_bit(used, p)) {
-                        const int owner = __float_as_int(S->plx[p].y);
-                        if (owner >= 0) {
-                            ids[k] = (int32_t)perm[owner];
-                            mine++;
-                        }
-                    }
-                }
-                int at = gexcl(mine, F);
-                if (F > row_cap) return false;
-#pragma unroll
-                for (int k = 0; k < kPRows; ++k)
-                    if (ids[k] >= 0) S->tmp[at++] = (uint32_t)ids[k];
                 // rank of each id among the row (ids are distinct)
-#pragma unroll
-                for (int k = 0; k < kPRows; ++k) {
-                    if (ids[k] < 0) continue;
+                for (int t = gl; t < F; t += G) {
+                    const int32_t x = (int32_t)S->tmp[t];
                     int rank = 0;
-                    for (int t = 0; t < F; ++t) rank += (int32_t)S->tmp[t] < ids[k];
-                    row_out[rank] = ids[k];
+                    for (int u = 0; u < F; ++u) rank += (int32_t)S->tmp[u] < x;
+                    row_out[rank] = x;
                 }
             }
         }
@@ -1186,7 +1233,6 @@ struct Group {
                                                        : 0.5f * fmaxf(fmaxf(fmaxf(-L_[0], U_[0]), fmaxf(-L_[1], U_[1])),
                                                                       fmaxf(-L_[2], U_[2]));
         }
-        return true;
     }
 
     // Does a final vertex lie on a plane of the predicted cube (group op)?
@@ -1204,6 +1250,9 @@ struct Group {
 
     __device__ __forceinline__ void start(uint32_t cell, const CellParams &prm) {
         seed = cell;
+        if constexpr (kExact) {
+            if (gl == 0) S->exact_events = 0u;
+        }
         const float4 s = pts[cell];
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
         status = kSuccess;
@@ -1264,7 +1313,7 @@ struct Group {
         inv = prm.inv;
         nodes = prm.nodes;
         set_cloud(prm.hdr);
-        if (gl < 12) S->ctr[gl] = 0u;
+        if (gl < kNumCtr) S->ctr[gl] = 0u;
         for (int w = gl; w < (int)(sizeof(S->pm) / 4); w += G) S->pm[w] = 0u;
     }
 
@@ -1275,12 +1324,12 @@ struct Group {
     }
 
     __device__ __forceinline__ void flush(const CellParams &prm, uint32_t cells) const {
-        uint32_t vals[12];
+        uint32_t vals[kNumCtr];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) vals[k] = gl == 0 ? S->ctr[k] : 0u;
+        for (int k = 0; k < kNumCtr; ++k) vals[k] = gl == 0 ? S->ctr[k] : 0u;
         vals[kStatCells] = cells;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
+        for (int k = 0; k < kNumCtr; ++k) {
             const uint32_t s = wave_sum(vals[k]);
             if (lane64 == 0 && s) atomicAdd(&prm.stats[k], (unsigned long long)s);
         }
@@ -1308,7 +1357,6 @@ constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Th
 // fewer than 64 levels: 128 entries never fill)
 constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 128, kT1Threads = 128;
 constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 128, kT2Threads = 64;
-static_assert(kT0Planes <= kRowCap && kT1Planes <= kTier1Row && kT2Planes <= kTier2Row, "rows fit their strides");
 using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, false>;
 using Tier0x = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, true>;
 using Tier1 = Group<kT1Lanes, kT1Verts, kT1Planes, kT1List, true>;
@@ -1329,6 +1377,36 @@ using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 __device__ __forceinline__ const CellParams *reread(const CellParams *p) {
     asm volatile("" : "+s"(p));
     return p;
+}
+
+// A row that does not fit the row pool: provisional status 1 and an empty
+// row, counted, with the pool size the launch needs (the host runs the
+// launch again with a pool that large).
+__device__ __forceinline__ void pool_short(const CellParams &P, uint32_t cell, uint32_t orig, uint64_t need) {
+    P.row_offsets[cell] = 0;
+    P.counts[orig] = 0;
+    P.status[orig] = kPlaneOverflow;
+    atomicAdd(&P.stats[kStatPoolShort], 1ull);
+    atomicMax(&P.stats[kStatPoolDemand], (unsigned long long)need);
+}
+// A cell beyond the largest tier: the reference's capacity status with an
+// empty row (counted, never silent).
+__device__ __forceinline__ void truncate_cell(const CellParams &P, uint32_t orig) {
+    P.counts[orig] = 0;
+    P.status[orig] = kPlaneOverflow;
+    atomicAdd(&P.stats[kStatTruncated], 1ull);
+}
+// Counts the removal tests of a cell that expansion arithmetic decided, and
+// records the cell's original id in the first free log slot.
+__device__ __forceinline__ void log_exact(const CellParams &P, uint32_t orig, uint32_t events) {
+    const unsigned long long k = atomicAdd(&P.stats[kStatExact], (unsigned long long)events);
+    if (k < (unsigned long long)kExactLogSlots) P.stats[kStatExactLog + k] = (unsigned long long)orig + 1ull;
+}
+// Takes n entries of the row pool (lane 0 asks; every lane gets the base).
+__device__ __forceinline__ uint64_t pool_take(const CellParams &P, uint32_t n) {
+    unsigned long long b = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0 && n > 0u) b = atomicAdd(P.pool_top, (unsigned long long)n);
+    return ((uint64_t)rl((uint32_t)(b >> 32), 0) << 32) | (uint64_t)rl((uint32_t)b, 0);
 }
 
 // kFlagged: the instance that takes the cells flagged by the first one
@@ -1356,37 +1434,40 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         const uint32_t cell = (uint32_t)(prm.begin + (active ? c : 0));
         if (active) g.start(cell, *reread(&sprm));
         g.traverse();
-        if (!active) return;
         // the launch's fields again from the argument segment (scalar loads)
         // rather than kept in registers across the traversal
         const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
-        bool done = false;
+        if (kExactTier && active && g.gl == 0 && g.S->exact_events) log_exact(P, orig, g.S->exact_events);
         // a cell built inside a predicted cube stands only if the cube cut nothing
-        if (!kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
+        if (active && !kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
             g.overflow = g.bail = true;
-        if (!g.overflow) {
-            if (g.gl == 0) P.row_offsets[cell] = c * P.row_stride;
-            done = g.emit(P, orig, P.rows + c * P.row_stride, P.row_stride);
-        }
-        if (!done && g.gl == 0) {
+        const bool emit = active && !g.overflow;
+        const int F = emit ? g.stage_row(P) : 0;
+        // one row-pool allocation for the wave's four rows
+        const int f0 = rl(F, 0), f1 = rl(F, 16), f2 = rl(F, 32), f3 = rl(F, 48);
+        const uint64_t base = pool_take(P, (uint32_t)(f0 + f1 + f2 + f3)) + (uint64_t)(grp > 0 ? f0 : 0) +
+                              (uint64_t)(grp > 1 ? f1 : 0) + (uint64_t)(grp > 2 ? f2 : 0);
+        if (emit) {
+            if (base + (uint64_t)F <= P.rows_elems) {
+                if (g.gl == 0) P.row_offsets[cell] = (int64_t)base;
+                g.write_row(P, orig, P.rows + base, F);
+            } else if (g.gl == 0) {
+                pool_short(P, cell, orig, base + (uint64_t)F);
+            }
+        } else if (active && g.gl == 0) {
             if (!kExactTier && g.bail) {  // flag for the FP64-capable instance
                 P.exact_list[c] = 1u;
                 atomicAdd(&P.counters[5], 1u);
             } else {  // queue for the large-capacity tier
-                g.count(kStatOverflow);
+                g.count(kStatPromoted);
                 const uint32_t slot = atomicAdd(&P.counters[1], 1u);
-                if (slot < P.big_cap) {
-                    P.big_list[slot] = cell;
-                } else {
-                    P.row_offsets[cell] = c * P.row_stride;
-                    P.counts[orig] = 0;
-                    P.status[orig] = kPlaneOverflow;
-                }
+                if (slot < P.queue_cap) P.big_list[slot] = cell;
+                else truncate_cell(P, orig);
             }
         }
-        if (P.cycles && g.gl == 0) P.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
-        if (g.gl == 0 && !g.bail) cells++;
+        if (active && P.cycles && g.gl == 0) P.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (active && g.gl == 0 && !g.bail) cells++;
     };
     if constexpr (!kExactTier) {
         if (prm.clouds == nullptr) {
@@ -1449,7 +1530,7 @@ __global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const Cell
     tier0_body<Tier0x, true>(prm);
 }
 
-// Tier 1: one cell per wave, the queued cells; rows after the tier-0 slab.
+// Tier 1: one cell per wave, the queued cells.
 __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT1Verts, kT1Planes> lds[kT1Threads / kWave];
     __shared__ CandLDS<kT1Lanes> cands[kT1Threads / kWave];
@@ -1459,7 +1540,7 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
     const int lane = threadIdx.x & (kWave - 1);
     Tier1 g;
     g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
-    const uint32_t queued = min(prm.counters[1], prm.big_cap);
+    const uint32_t queued = min(prm.counters[1], prm.queue_cap);
     while (true) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&prm.counters[2], 1u);
@@ -1473,19 +1554,21 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         g.traverse();
         const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
-        const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)k * kTier1Row;
-        if (lane == 0) P.row_offsets[cell] = off;
-        if (g.overflow || !g.emit(P, orig, P.rows + off, kTier1Row)) {
-            if (lane == 0) {  // queue for tier 2
-                g.count(kStatOverflow);
-                const uint32_t slot = atomicAdd(&P.counters[3], 1u);
-                if (slot < P.huge_cap) {
-                    P.huge_list[slot] = cell;
-                } else {
-                    P.counts[orig] = 0;
-                    P.status[orig] = kPlaneOverflow;
-                }
+        if (lane == 0 && g.S->exact_events) log_exact(P, orig, g.S->exact_events);
+        if (!g.overflow) {
+            const int F = g.stage_row(P);
+            const uint64_t base = pool_take(P, (uint32_t)F);
+            if (base + (uint64_t)F <= P.rows_elems) {
+                if (lane == 0) P.row_offsets[cell] = (int64_t)base;
+                g.write_row(P, orig, P.rows + base, F);
+            } else if (lane == 0) {
+                pool_short(P, cell, orig, base + (uint64_t)F);
             }
+        } else if (lane == 0) {  // queue for tier 2
+            g.count(kStatPromoted);
+            const uint32_t slot = atomicAdd(&P.counters[3], 1u);
+            if (slot < P.queue_cap) P.huge_list[slot] = cell;
+            else truncate_cell(P, orig);
         }
         if (P.cycles && lane == 0)
             P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
@@ -1494,7 +1577,7 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
 }
 
 // Tier 2: one cell per wave, one wave per block (57 KB of LDS), the cells
-// that outgrow tier 1; rows after the tier-1 rows.
+// that outgrow tier 1.
 __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellParams prm) {
     __shared__ GroupLDS<kT2Verts, kT2Planes> lds;
     __shared__ CandLDS<kT2Lanes> cands;
@@ -1504,7 +1587,7 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
     const int lane = threadIdx.x & (kWave - 1);
     Tier2 g;
     g.setup(prm, &lds, &cands);
-    const uint32_t queued = min(prm.counters[3], prm.huge_cap);
+    const uint32_t queued = min(prm.counters[3], prm.queue_cap);
     while (true) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
@@ -1518,17 +1601,18 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
         g.traverse();
         const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
-        const int64_t off = (int64_t)(P.end - P.begin) * P.row_stride + (int64_t)P.big_cap * kTier1Row +
-                            (int64_t)k * kTier2Row;
-        if (lane == 0) P.row_offsets[cell] = off;
-        if (g.overflow || !g.emit(P, orig, P.rows + off, kTier2Row)) {
-            // beyond every tier: report the reference's capacity status with
-            // an empty row (counted, never silent)
-            if (lane == 0) {
-                g.count(kStatOverflow);
-                P.counts[orig] = 0;
-                P.status[orig] = kPlaneOverflow;
+        if (lane == 0 && g.S->exact_events) log_exact(P, orig, g.S->exact_events);
+        if (!g.overflow) {
+            const int F = g.stage_row(P);
+            const uint64_t base = pool_take(P, (uint32_t)F);
+            if (base + (uint64_t)F <= P.rows_elems) {
+                if (lane == 0) P.row_offsets[cell] = (int64_t)base;
+                g.write_row(P, orig, P.rows + base, F);
+            } else if (lane == 0) {
+                pool_short(P, cell, orig, base + (uint64_t)F);
             }
+        } else if (lane == 0) {
+            truncate_cell(P, orig);  // beyond every tier
         }
         if (P.cycles && lane == 0)
             P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);

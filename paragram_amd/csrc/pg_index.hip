@@ -392,34 +392,158 @@ __global__ void refit_top_kernel(const float4 *__restrict__ pts, int64_t n, cons
 
 // Points identical in position and weight have identical Morton keys, so
 // each duplicate group lies inside one run of equal keys: scan the run.
-__global__ void duplicates_kernel(const uint32_t *__restrict__ keys, const float4 *__restrict__ pts, int64_t n,
-                                  uint32_t *__restrict__ head, uint32_t *__restrict__ next,
-                                  IndexHeader *__restrict__ hdr) {
+// Groups of exact duplicates (equal position and weight) among the
+// Morton-sorted points, for the rows (every duplicate of a face owner is a
+// neighbour): head[i] is the smallest sorted index of i's group and next[]
+// chains every member from the head (0xFFFFFFFF ends a chain).  Duplicates
+// share their Morton key, so only equal-key runs need a look.  A run of at
+// most kDupWindow points is scanned by each member (the chain then runs in
+// sorted order); the members of a longer run (a cluster denser than the
+// Morton grid, or many coincident points) go through a hash table instead,
+// so the work stays linear whatever the run length.
+constexpr int kDupWindow = 32;
+constexpr uint32_t kDupLong = 0xFFFFFFFEu;  // head[] mark of a member of a long run
+constexpr uint32_t kDupEnd = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool same_point(float4 a, float4 b) {
+    return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+// Hash consistent with same_point (-0 and +0 compare equal, so both hash as +0).
+__device__ __forceinline__ uint32_t point_hash(float4 p) {
+    const float v[4] = {p.x == 0.0f ? 0.0f : p.x, p.y == 0.0f ? 0.0f : p.y, p.z == 0.0f ? 0.0f : p.z,
+                        p.w == 0.0f ? 0.0f : p.w};
+    uint32_t h = 0x9E3779B9u;
+    for (int k = 0; k < 4; ++k) {
+        h = (h ^ __float_as_uint(v[k])) * 0x85EBCA6Bu;
+        h ^= h >> 13;
+    }
+    return h * 0xC2B2AE35u;
+}
+__device__ __forceinline__ uint32_t dup_slot(uint32_t h, uint32_t table_size) {
+    return (uint32_t)(((uint64_t)h * table_size) >> 32);
+}
+
+__global__ void dup_runs_kernel(const uint32_t *__restrict__ keys, const float4 *__restrict__ pts, int64_t n,
+                                uint32_t *__restrict__ head, uint32_t *__restrict__ next,
+                                IndexHeader *__restrict__ hdr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i];
+    // the run [s, e] of key k around i, as far as the window reaches; every
+    // member of a run reaches the same verdict: longer than the window or not
+    int64_t s = i, e = i;
+    for (int t = 0; t < kDupWindow && s > 0 && keys[s - 1] == k; ++t) --s;
+    bool lng = s > 0 && keys[s - 1] == k;
+    for (int t = 0; t < kDupWindow && !lng && e + 1 < n && keys[e + 1] == k; ++t) ++e;
+    lng = lng || (e + 1 < n && keys[e + 1] == k) || e - s + 1 > kDupWindow;
+    if (lng) {
+        head[i] = kDupLong;
+        next[i] = kDupEnd;
+        return;
+    }
     const float4 p = pts[i];
     int64_t h = i;
-    for (int64_t j = i - 1; j >= 0 && keys[j] == k; --j) {
-        const float4 q = pts[j];
-        if (q.x == p.x && q.y == p.y && q.z == p.z && q.w == p.w) h = j;
+    for (int64_t j = s; j < i; ++j) {
+        if (same_point(pts[j], p)) {
+            h = j;
+            break;
+        }
     }
-    uint32_t nx = 0xFFFFFFFFu;
-    for (int64_t j = i + 1; j < n && keys[j] == k; ++j) {
-        const float4 q = pts[j];
-        if (q.x == p.x && q.y == p.y && q.z == p.z && q.w == p.w) {
+    uint32_t nx = kDupEnd;
+    for (int64_t j = i + 1; j <= e; ++j) {
+        if (same_point(pts[j], p)) {
             nx = (uint32_t)j;
             break;
         }
     }
     head[i] = (uint32_t)h;
     next[i] = nx;
-    if (h != i || nx != 0xFFFFFFFFu) hdr->has_duplicates = 1u;
+    if (h != i || nx != kDupEnd) hdr->has_duplicates = 1u;
+}
+
+// Members of long runs: the table (table_size = 2n slots, all kDupEnd) maps
+// each distinct point to the smallest index of its group; linear probing,
+// and the table never holds more than n entries, so a probe ends.
+__global__ void dup_insert_kernel(const float4 *__restrict__ pts, int64_t n, const uint32_t *__restrict__ head,
+                                  uint32_t *__restrict__ table, uint32_t table_size) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || head[i] != kDupLong) return;
+    const float4 p = pts[i];
+    uint32_t sl = dup_slot(point_hash(p), table_size);
+    for (uint32_t probe = 0; probe < table_size; ++probe) {
+        uint32_t v = table[sl];
+        if (v == kDupEnd) {
+            v = atomicCAS(&table[sl], kDupEnd, (uint32_t)i);
+            if (v == kDupEnd) return;
+        }
+        // any member identifies the group; keep the smallest index
+        if (same_point(pts[v], p)) {
+            atomicMin(&table[sl], (uint32_t)i);
+            return;
+        }
+        sl = sl + 1 == table_size ? 0 : sl + 1;
+    }
+}
+
+// Members of long runs: head = the group's smallest index; every other
+// member links itself in right after the head (next[head] was set to the
+// end by dup_runs_kernel).
+__global__ void dup_link_kernel(const float4 *__restrict__ pts, int64_t n, uint32_t *__restrict__ head,
+                                uint32_t *__restrict__ next, const uint32_t *__restrict__ table, uint32_t table_size,
+                                IndexHeader *__restrict__ hdr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || head[i] != kDupLong) return;
+    const float4 p = pts[i];
+    uint32_t sl = dup_slot(point_hash(p), table_size);
+    uint32_t h = (uint32_t)i;
+    for (uint32_t probe = 0; probe < table_size; ++probe) {
+        const uint32_t v = table[sl];
+        if (v == kDupEnd) break;  // (not reached: i itself was inserted)
+        if (same_point(pts[v], p)) {
+            h = v;
+            break;
+        }
+        sl = sl + 1 == table_size ? 0 : sl + 1;
+    }
+    head[i] = h;
+    if (h != (uint32_t)i) {
+        next[i] = atomicExch(&next[h], (uint32_t)i);
+        hdr->has_duplicates = 1u;
+    }
 }
 
 __global__ void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Leaf statistics of the hierarchy (the debug print of pwr_bvh.cu:342-453):
+// out = {leaves, fewest points in a leaf, most, points in all leaves}.
+__global__ void leaf_stats_init_kernel(unsigned long long *out) {
+    if (threadIdx.x < 4) out[threadIdx.x] = threadIdx.x == 1 ? ~0ull : 0ull;
+}
+// A node whose range holds at most leaf_size points was collapsed into a
+// leaf of its parent (its record is written but never visited); ranges grow
+// towards the root, so a node is visited iff its range is larger.
+__global__ void leaf_stats_kernel(const Node *__restrict__ nodes, const uint2 *__restrict__ ranges, int64_t n_internal,
+                                  const IndexHeader *__restrict__ hdr, unsigned long long *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t refs[2] = {0u, 0u};
+    int m = 0;
+    if (i < n_internal && ranges[i].y - ranges[i].x + 1u > hdr->leaf_size) {
+        refs[m++] = __float_as_uint(nodes[i].hi0.w);
+        refs[m++] = __float_as_uint(nodes[i].hi1.w);
+    } else if (i == 0 && is_leaf(hdr->root_ref)) {
+        refs[m++] = hdr->root_ref;  // no more points than a leaf holds: the root is a leaf
+    }
+    for (int k = 0; k < m; ++k) {
+        if (!is_leaf(refs[k])) continue;
+        const unsigned long long c = leaf_count(refs[k]);
+        atomicAdd(&out[0], 1ull);
+        atomicMin(&out[1], c);
+        atomicMax(&out[2], c);
+        atomicAdd(&out[3], c);
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -470,8 +594,16 @@ static hipError_t build_tail(int64_t n, const float *points, const float *weight
     }
     hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, weights, n, perm, sorted,
                        inv);
-    hipLaunchKernelGGL(duplicates_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, keys, sorted, n, dup_head,
+    const uint32_t table_size = (uint32_t)(2 * n);
+    uint32_t *dup_table = (uint32_t *)(base + L.dup_table);
+    hipLaunchKernelGGL(dup_runs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, keys, sorted, n, dup_head,
                        dup_next, hdr);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(table_size, 256)), dim3(256), 0, stream, dup_table,
+                       (int64_t)table_size, kDupEnd);
+    hipLaunchKernelGGL(dup_insert_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted, n, dup_head,
+                       dup_table, table_size);
+    hipLaunchKernelGGL(dup_link_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted, n, dup_head, dup_next,
+                       dup_table, table_size, hdr);
     if (n >= 2) {
         hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, stream, keys, n, children, parents,
                            ranges);
@@ -485,6 +617,17 @@ static hipError_t build_tail(int64_t n, const float *points, const float *weight
     return hipGetLastError();
 }
 
+
+hipError_t leaf_stats(int64_t n, const void *ws, unsigned long long *out, hipStream_t stream) {
+    const IndexLayout L = make_index_layout(n);
+    const char *base = (const char *)ws;
+    const int64_t ni = n >= 2 ? n - 1 : 0;
+    hipLaunchKernelGGL(leaf_stats_init_kernel, dim3(1), dim3(64), 0, stream, out);
+    hipLaunchKernelGGL(leaf_stats_kernel, dim3(grid_for(ni > 0 ? ni : 1, 256)), dim3(256), 0, stream,
+                       (const Node *)(base + L.nodes), (const uint2 *)(base + L.ranges), ni,
+                       (const IndexHeader *)(base + L.header), out);
+    return hipGetLastError();
+}
 
 hipError_t build_index(int64_t n, const float *points, const float *weights, int leaf_size, void *ws,
                        hipStream_t stream) {

@@ -15,6 +15,7 @@ hipError_t build_index(int64_t n, const float *points, const float *weights, int
                        hipStream_t stream);
 hipError_t build_batch_index(int64_t n, int n_clouds, const float *points, const float *weights,
                              const int64_t *cloud_offsets, int leaf_size, void *ws, hipStream_t stream);
+hipError_t leaf_stats(int64_t n, const void *ws, unsigned long long *out, hipStream_t stream);
 hipError_t knn(int64_t n, void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
                int n_clouds = 0, bool sorted_rows = false);
 hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
@@ -34,18 +35,16 @@ struct CellArgs {
     int guess_stride = 0;         // > 0: Morton-ordered rows of this many sorted ids (offsets unused)
     const float *guess_d2 = nullptr;  // with guess_stride: squared distance of each row's last id (sorted order)
     int seed_window;              // Morton neighbours offered before the traversal
-    int32_t *rows;                // (end - begin) x row_stride, big_cap x kTier1Row, huge_cap x kTier2Row
-    int row_stride;
+    int32_t *rows;                // row pool: every built cell's row, bump-allocated
+    int64_t rows_elems;           // capacity of the row pool
     int64_t *row_offsets;         // [n] sorted cell -> row start in rows
     int32_t *counts;              // [n] original order
     int32_t *status;              // [n] original order
     unsigned long long *stats;    // kNumStats counters (accumulated)
     void *scratch;
     uint64_t scratch_bytes;
-    int waves;                    // persistent tier-0 waves (scratch holds one slot per wave)
+    int waves;                    // persistent tier-0 waves
     int waves1;                   // persistent tier-1 waves
-    int64_t big_cap;              // tier-1 queue capacity
-    int64_t huge_cap;             // tier-2 queue capacity
     uint32_t *cycles;             // optional per-cell cost counters
     const IndexHeader *clouds = nullptr;  // batch: per-cloud headers (null: one cloud)
     const int64_t *cloud_cells = nullptr; // batch: [n_clouds + 1] sorted-cell offsets
@@ -55,7 +54,9 @@ struct CellArgs {
     float *extent = nullptr;              // optional built half-width per cell (original order)
 };
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream);
-uint64_t cell_scratch_bytes(int waves, int waves1, int64_t big_cap, int64_t huge_cap, int64_t cells);
+hipError_t exact_side(int64_t n, const void *index_ws, int64_t m, const int32_t *queries, int32_t *out,
+                      hipStream_t stream);
+uint64_t cell_scratch_bytes(int64_t cells);
 
 // CSR helpers.
 hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int64_t begin, int64_t end,

@@ -43,27 +43,30 @@ class GraphedDiagram:
         waves = ctypes.c_int32(0)
         lib.call("pg_default_waves", ctypes.byref(waves))
         self.waves = int(waves.value)
-        self.scratch_bytes = lib.u64("pg_cell_scratch_bytes", self.waves, n)
+        self.scratch_bytes = lib.u64("pg_cell_scratch_bytes", n)
         self.scratch = torch.empty(self.scratch_bytes, dtype=torch.uint8, device=dev)
-        cap = lib.u64("pg_row_buffer_elems", n)
+        cap = lib.u64("pg_row_pool_elems", n)
         self.rows = torch.empty(cap, **i32)
         self.row_offsets = torch.empty(n, dtype=torch.int64, device=dev)
         self.counts = torch.empty(n, **i32)
         self.status = torch.empty(n, **i32)
         self.stats = torch.empty(_engine.NUM_STATS, dtype=torch.int64, device=dev)
         self.offsets = torch.empty(n + 1, **i32)
-        self.adjacency = torch.empty(max(cap, 1), **i32)  # every row fits: E <= the row buffer
+        self.adjacency = torch.empty(max(cap, 1), **i32)  # every row fits: E <= the row pool
         self.scan_tmp = torch.empty(max(lib.u64("pg_scan_tmp_bytes", n + 1), 4), dtype=torch.uint8, device=dev)
         self.stride = _engine.KNN_PRECLIP if knn > 0 else 0
         self.guesses = torch.empty(max(n * self.stride, 1), **i32)
         self.guess_d2 = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
         self.leaf_size = leaf_size
+        self._capture()
+
+    def _capture(self):
         self._pipeline()  # once eagerly (first-use work outside the capture)
-        torch.cuda.synchronize(dev)
+        torch.cuda.synchronize(self.dev)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self._pipeline()
-        torch.cuda.synchronize(dev)
+        torch.cuda.synchronize(self.dev)
 
     def _pipeline(self):
         lib, n = self.lib, self.n
@@ -76,7 +79,7 @@ class GraphedDiagram:
             lib.call("pg_knn_guesses", n, _p(self.ws), 0, self.stride, _p(self.guesses), _p(self.guess_d2), sp)
             g, gd = self.guesses, self.guess_d2
         lib.call("pg_compute_cells_ex", n, _p(self.ws), 0, n, _p(g), None, 0, self.stride, _p(gd), 0, None, None,
-                 _p(self.rows), _p(self.row_offsets), _p(self.counts), _p(self.status), _p(self.stats),
+                 _p(self.rows), self.rows.numel(), _p(self.row_offsets), _p(self.counts), _p(self.status), _p(self.stats),
                  _p(self.scratch), self.scratch_bytes, self.waves, None, sp)
         lib.call("pg_scan_counts", n, _p(self.counts), _p(self.offsets), _p(self.scan_tmp), sp)
         lib.call("pg_check_symmetry", n, _p(self.ws), _p(self.counts), _p(self.rows), _p(self.row_offsets), 0,
@@ -96,6 +99,17 @@ class GraphedDiagram:
             self.weights.copy_(weights.reshape(self.n, 1))
         self.graph.replay()
         st = self.stats.cpu()  # the one device-to-host read
+        if int(st[_engine.STAT_POOL_SHORT]):
+            # the rows did not fit the row pool: grow it to what the launch
+            # reported, capture again and replay (the only allocating path)
+            need = int(st[_engine.STAT_POOL_DEMAND])
+            i32 = dict(dtype=torch.int32, device=self.dev)
+            self.rows = self.adjacency = None
+            self.rows = torch.empty(need + need // 8 + 65536, **i32)
+            self.adjacency = torch.empty(self.rows.numel(), **i32)
+            self._capture()
+            self.graph.replay()
+            st = self.stats.cpu()
         e = int(st[15])
         adjacency, offsets = self.adjacency[:e], self.offsets
         if int(st[8]) or int(st[9]):

@@ -156,3 +156,32 @@ def test_trained_network_gain(gpu):
     assert work_c < work_p
     d = learned.power_diagram(P, W, model=model)
     assert torch.equal(d.adjacency, plain[0]) and torch.equal(d.offsets, plain[1])
+
+
+def test_c5_learned_form_with_committed_model(gpu):
+    """The learned-radius power configuration (10M points, seed 4) with the
+    committed network (paragram_amd/radius_mlp.npz): byte-identical to the
+    plain call, and a seeded 100k-cell sample equal to the oracle."""
+    import torch
+
+    import oracle
+    import paragram_amd
+    from paragram_amd import learned
+
+    log("C5 10M: learned form (committed radius_mlp.npz) vs plain vs 100k-cell oracle sample")
+    pts, w = workloads.learned_power(10_000_000, 4)
+    P, W = torch.from_numpy(pts).to(gpu), torch.from_numpy(w).to(gpu)
+    model = learned.load()
+    d = learned.power_diagram(P, W, model=model)
+    got = (d.adjacency.cpu().numpy(), d.offsets.cpu().numpy(), d.status.cpu().numpy())
+    del d
+    plain = paragram_amd.power_diagram(P, W)
+    for name, x, y in zip(("adjacency", "offsets", "status"), got, (plain.adjacency, plain.offsets, plain.status)):
+        assert np.array_equal(x, y.cpu().numpy()), f"learned form: {name} differs from the plain call"
+    del plain
+    adj, off, st = got
+    sample = workloads.oracle_sample(pts.shape[0], 4, 100_000)
+    ref = oracle.cells(pts, w, sample)
+    bad = sum(int(st[i] != ref.status[k] or not np.array_equal(adj[off[i]:off[i + 1]], ref.row(k)))
+              for k, i in enumerate(sample))
+    assert bad == 0, f"{bad} of {sample.shape[0]} sampled cells differ"

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from paragram_amd import workloads
-from tests.helpers import csr_invariants
+from tests.helpers import csr_invariants, csr_invariants_fast
 
 pytestmark = pytest.mark.gpu
 
@@ -204,3 +204,63 @@ def test_cubic_lattice_is_face_adjacency(gpu, weighted):
     assert (st == 0).all()
     assert np.array_equal(off, want_off), "row lengths differ from the lattice's face counts"
     assert np.array_equal(adj, want_adj), "rows differ from the lattice's face neighbours"
+
+
+def test_row_pool_growth_gives_identical_output(gpu):
+    """A row pool far too small for the rows: the launch reports the size it
+    needs and runs again; the output equals the default call's, and no cell
+    is left truncated."""
+    import torch
+
+    from paragram_amd import _engine
+    from paragram_amd._backend import _C
+
+    log("row pool growth: 20k uniform and 20k power, pool of 1000 entries, unchunked and chunked")
+    pts = workloads.uniform(20_000, 21)
+    rng = np.random.default_rng(21)
+    w = rng.uniform(0, 2e-4, 20_000).astype(np.float32)
+    for weights in (None, w):
+        p = torch.from_numpy(pts).to(gpu)
+        wt = None if weights is None else torch.from_numpy(weights).reshape(-1, 1).to(gpu)
+        want = [t.cpu().numpy() for t in _engine.build_diagram(_C.lib, p, wt)]
+        for chunks in (None, "3"):
+            if chunks:
+                os.environ["PARAGRAM_NUM_CHUNKS"] = chunks
+            try:
+                ins = _engine.Instrument()
+                got = [t.cpu().numpy() for t in _engine.build_diagram(_C.lib, p, wt, row_pool=1000, instrument=ins)]
+            finally:
+                os.environ.pop("PARAGRAM_NUM_CHUNKS", None)
+            for x, y in zip(got, want):
+                assert np.array_equal(x, y)
+            assert ins.stats["pool_short"] == 0 and ins.stats["truncated"] == 0
+            assert len(ins.cell_events) >= 2, "the small pool must have made a launch run again"
+
+
+def test_long_equal_key_runs_cluster_and_outlier(gpu):
+    """A dense cluster with one far outlier puts most points under a handful
+    of Morton keys (runs of thousands): the duplicate search must stay linear
+    and exact.  200k clustered points (with 2,000 exact duplicates) plus an
+    outlier 1e4 cluster sizes away, against the oracle."""
+    log("long equal-key runs: 200k cluster + outlier at 1e4 x the cluster size, with duplicates")
+    rng = np.random.default_rng(31)
+    c = (0.5 + 1e-4 * rng.standard_normal((200_000, 3))).astype(np.float32)
+    c[-2000:] = c[:2000]  # exact duplicates inside the long runs
+    pts = np.concatenate([c, np.array([[0.5 + 4.0, 0.5, 0.5]], np.float32)])
+    got = run(gpu, pts)
+    csr_invariants_fast(*got)
+    assert_matches_oracle(got, pts)
+
+
+def test_all_points_coincide(gpu):
+    """Every point at the same position (zero extent: every Morton key equal)
+    plus three distinct points, unweighted and with distinct weights."""
+    log("coincident points: 5,000 copies of one point + 3 others")
+    pts = np.concatenate([np.full((5_000, 3), 0.25, np.float32),
+                          np.array([[0.5, 0.5, 0.5], [0.9, 0.1, 0.3], [0.2, 0.8, 0.7]], np.float32)])
+    got = run(gpu, pts)
+    assert_matches_oracle(got, pts)
+    rng = np.random.default_rng(2)
+    w = rng.uniform(0, 1e-3, pts.shape[0]).astype(np.float32)
+    got = run(gpu, pts, w)
+    assert_matches_oracle(got, pts, w)

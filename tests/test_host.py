@@ -159,7 +159,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.pg_abi_version() == 1
+    assert lib.pg_abi_version() == 2
 
 
 def test_library_argument_errors_without_gpu():
@@ -174,9 +174,18 @@ def test_library_argument_errors_without_gpu():
     assert b"non-negative" in lib.lib.pg_last_error()
     assert lib.lib.pg_index_workspace_bytes(1 << 27, ctypes.byref(out)) == 1
     assert lib.lib.pg_index_workspace_bytes(1000, ctypes.byref(out)) == 0 and out.value > 1000 * 16
-    assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, 16, None, None, None, None, None, None, 0, 1,
-                                    None, None) == 1
+    assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, 16, None, 0, None, None, None, None, None, 0,
+                                    1, None, None) == 1
     assert b"out of bounds" in lib.lib.pg_last_error()
+    # the row pool and the cell scratch are sized per cell (12 bytes of
+    # scratch: the FP64 flag and both queues, so no queue can fill)
+    assert lib.lib.pg_row_pool_elems(1000, ctypes.byref(out)) == 0 and out.value == 32 * 1000 + 65536
+    assert lib.lib.pg_cell_scratch_bytes(1000, ctypes.byref(out)) == 0 and out.value == 256 + 12 * 1000
+    # the index layout (host only): increasing offsets inside the workspace
+    offs = (ctypes.c_uint64 * 6)()
+    assert lib.lib.pg_index_layout(1000, offs) == 0
+    assert list(offs) == sorted(offs) and offs[0] == 0
+    assert lib.lib.pg_index_workspace_bytes(1000, ctypes.byref(out)) == 0 and out.value == offs[5]
 
 
 # ---------------------------------------------------------------- batches
@@ -224,3 +233,37 @@ def test_batch_api_validation():
     with pytest.raises(ValueError, match="1-D integer"):
         _api._validate_cloud_offsets([0.0, 7.0], 7)
     assert list(_api._validate_cloud_offsets(torch.tensor([0, 2, 7]), 7)) == [0, 2, 7]
+
+
+def test_lazy_build_on_first_import(monkeypatch, tmp_path, capsys):
+    """A missing library is built on first load (as the reference's JIT build
+    on first import, paragram/_backend.py:147-190), a present one is loaded
+    as is, and PARAGRAM_NO_BUILD=1 leaves the engine unavailable instead.
+    The builder is replaced by a copy of the built library (the real hipcc
+    build is what __graft_entry__.build() runs)."""
+    import shutil
+
+    from paragram_amd import _backend, _build, _engine
+
+    if not os.path.exists(_engine.LIB_PATH):
+        pytest.skip("libparagram_hip.so not built")
+    monkeypatch.setattr(_backend, "_load_error", _backend._load_error)
+    calls = []
+
+    def fake_build(verbose=None, lib_path=_build.LIB_PATH):
+        calls.append(lib_path)
+        shutil.copy(_engine.LIB_PATH, lib_path)
+        return lib_path
+
+    monkeypatch.setattr(_build, "build", fake_build)
+    monkeypatch.delenv("PARAGRAM_NO_BUILD", raising=False)
+    target = str(tmp_path / "libparagram_hip.so")
+    be = _backend._load(target)
+    assert be is not None and calls == [target]
+    out = capsys.readouterr().out
+    assert "building the HIP library for gfx950" in out and "HIP library ready in" in out
+    calls.clear()
+    assert _backend._load(target) is not None and calls == []
+    monkeypatch.setenv("PARAGRAM_NO_BUILD", "1")
+    assert _backend._load(str(tmp_path / "missing.so")) is None
+    assert "PARAGRAM_NO_BUILD" in _backend._load_error and calls == []

@@ -28,6 +28,8 @@ from scipy.spatial import ConvexHull, Delaunay
 
 import oracle
 from paragram_amd import workloads
+from tests.helpers import exact_side_fractions as _exact_side
+from tests.helpers import csr_invariants_fast, exact_tie_cloud
 from tests.helpers import csr_invariants, rows_of
 
 
@@ -203,41 +205,6 @@ def test_duplicates_are_canonical():
             assert k in rows[v] and 200 + k in rows[v]
 
 
-def _exact_side(points, weights, seed, tri, q):
-    """sign(P) * sign(D) of the removal predicate with Python Fractions."""
-    pts = np.asarray(points, np.float32)
-    w = np.zeros(len(pts), np.float32) if weights is None else np.asarray(weights, np.float32)
-    lo, hi = oracle.box(pts)
-    s = [Fraction(float(v)) for v in pts[seed]]
-    ws = Fraction(float(w[seed]))
-
-    def plane(o):
-        if o >= 0:
-            a = [Fraction(float(pts[o][k])) - s[k] for k in range(3)]
-            return a, sum(x * x for x in a) - (Fraction(float(w[o])) - ws)
-        code = -1 - o
-        axis, side = code // 2, code % 2
-        a = [Fraction(0)] * 3
-        if side == 0:
-            a[axis] = Fraction(-1)
-            return a, 2 * (s[axis] - Fraction(float(lo[axis])))
-        a[axis] = Fraction(1)
-        return a, 2 * (Fraction(float(hi[axis])) - s[axis])
-
-    (a1, h1), (a2, h2), (a3, h3) = (plane(t) for t in tri)
-    aq, hq = plane(q)
-
-    def cross(u, v):
-        return [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
-
-    c1, c2, c3 = cross(a2, a3), cross(a3, a1), cross(a1, a2)
-    D = sum(x * y for x, y in zip(a1, c1))
-    N = [h1 * c1[k] + h2 * c2[k] + h3 * c3[k] for k in range(3)]
-    P = sum(x * y for x, y in zip(aq, N)) - hq * D
-    sgn = lambda x: (x > 0) - (x < 0)
-    return sgn(P) * sgn(D), sgn(D), sgn(P)
-
-
 def test_exact_predicate_matches_fractions_random():
     rng = np.random.default_rng(3)
     pts = rng.random((40, 3)).astype(np.float32)
@@ -249,18 +216,17 @@ def test_exact_predicate_matches_fractions_random():
 
 
 def test_exact_predicate_near_degenerate():
-    # five points on a common sphere up to one float ulp: the FP64 filter
-    # cannot decide these, the expansion fallback must match the rationals
+    # exact ties (five points exactly co-spherical in FP32, or
+    # power-co-spherical with weights): P = 0, which the FP64 filter cannot
+    # decide, and one float ulp off them; the expansion fallback must match
+    # the rationals
     rng = np.random.default_rng(9)
     for trial in range(60):
-        ang = rng.random((5, 2)) * np.pi * [1, 2]
-        sph = np.stack([np.sin(ang[:, 0]) * np.cos(ang[:, 1]), np.sin(ang[:, 0]) * np.sin(ang[:, 1]),
-                        np.cos(ang[:, 0])], 1)
-        pts = (0.5 + 0.3 * sph).astype(np.float32)
+        pts, w = exact_tie_cloud(rng, weighted=trial % 2 == 1)
         if trial % 3 == 0:
             pts[4] = np.nextafter(pts[4], np.float32(2.0))
-        got = oracle.exact_side(pts, None, 0, [1, 2, 3], 4)
-        assert got == _exact_side(pts, None, 0, [1, 2, 3], 4)
+        got = oracle.exact_side(pts, w, 0, [1, 2, 3], 4)
+        assert got == _exact_side(pts, w, 0, [1, 2, 3], 4)
 
 
 def test_exact_predicate_box_planes():
@@ -280,3 +246,33 @@ def test_planted_mistakes_are_caught():
     flipped = r.status.copy()
     flipped[3] = 3
     assert not np.array_equal(flipped, np.zeros_like(flipped))
+
+
+def test_fast_csr_invariants_catch_planted_mistakes():
+    """The vectorised CSR check (used on the 20M-point batch) accepts the
+    oracle's diagram and rejects planted mistakes: a dropped reverse edge, an
+    unsorted row, a self loop, a row for a hidden cell."""
+    pts = workloads.uniform(400, 5)
+    adj, off, st, _ = oracle.diagram(pts)
+    csr_invariants_fast(adj, off, st)
+    bad = adj.copy()
+    bad[off[10]] = bad[off[10] + 1]  # duplicate in row 10: not strictly increasing, edge missing
+    with pytest.raises(AssertionError):
+        csr_invariants_fast(bad, off, st)
+    bad = adj.copy()
+    bad[off[10]], bad[off[10] + 1] = bad[off[10] + 1], bad[off[10]]  # row 10 unsorted
+    with pytest.raises(AssertionError):
+        csr_invariants_fast(bad, off, st)
+    bad = adj.copy()
+    bad[off[12]] = 12 if 12 < bad[off[12] + 1] else bad[off[12]]  # self loop in row 12
+    with pytest.raises(AssertionError):
+        csr_invariants_fast(bad, off, st)
+    flipped = st.copy()
+    flipped[7] = 3  # hidden, but with a row and listed by its neighbours
+    with pytest.raises(AssertionError):
+        csr_invariants_fast(adj, off, flipped)
+    drop = np.delete(adj, off[20])  # one directed edge gone: its reverse has no partner
+    off2 = off.copy()
+    off2[21:] -= 1
+    with pytest.raises(AssertionError):
+        csr_invariants_fast(drop, off2, st)
