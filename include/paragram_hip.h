@@ -134,6 +134,24 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats,
                      void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream);
 
+/* Largest cell range (end - begin) of pg_compute_cells_noalloc:
+ * 1,398,080 cells (its static cell scratch holds 16 MiB, 12 bytes a cell). */
+int pg_noalloc_max_cells(int64_t *cells);
+
+/* pg_compute_cells_ex without the scratch arguments: the cell scratch is a
+ * static 16 MiB array of the library on each GPU (held until the process
+ * ends), so the call allocates nothing (no hipMalloc, hipFree or allocating
+ * library routine); the caller still passes the inputs and outputs.  Ranges
+ * of more than pg_noalloc_max_cells cells are refused (split them).  Calls
+ * share the current GPU's array: calls that overlap on one GPU race.
+ * No counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it). */
+int pg_compute_cells_noalloc(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                             const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
+                             const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
+                             int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts,
+                             int32_t *status, uint64_t *stats, int32_t waves, uint32_t *cell_cycles, void *stream);
+
 /* The library's own kNN pre-clip: row i (Morton-sorted order) of ids holds
  * the sorted indices of point i's k nearest neighbours (itself as padding);
  * n_clouds = 0 for the index of pg_build_index, else the batch's count.

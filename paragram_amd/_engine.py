@@ -59,6 +59,9 @@ class HipLibrary:
             "pg_exact_side_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_index_layout": ([i64, u64p], ctypes.c_int),
             "pg_leaf_stats": ([i64, vp, vp, vp], ctypes.c_int),
+            "pg_noalloc_max_cells": ([ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+            "pg_compute_cells_noalloc": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, i32, vp, vp, vp, u64, vp, vp, vp,
+                                          vp, i32, vp, vp], ctypes.c_int),
             "pg_scan_counts": ([i64, vp, vp, vp, vp], ctypes.c_int),
             "pg_chunk_counts": ([i64, vp, i64, i64, vp, vp, vp], ctypes.c_int),
             "pg_pack_rows": ([i64, vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),

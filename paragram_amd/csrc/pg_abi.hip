@@ -6,6 +6,7 @@
 #include "pg_exact.hpp"
 #include "pg_kernels.hpp"
 
+#include <mutex>
 #include <stdio.h>
 #include <string>
 #include <vector>
@@ -29,6 +30,36 @@ inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 int check_n(int64_t n) {
     if (n < 0) return fail(1, "number of points must be non-negative");
     if (n > pg::kMaxPoints) return fail(1, "too many points for one call (limit 67108863)");
+    return 0;
+}
+
+// Cell scratch of pg_compute_cells_noalloc: one array per GPU, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// the device array: its device address comes from hipGetSymbolAddress, once
+// per device, and only that address is handed to kernels.
+constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on the first
+// call on that device, with that device current).
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return check_hip(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(1, "device ordinal out of range for the static cell scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(2, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
     return 0;
 }
 
@@ -433,6 +464,31 @@ int pg_symmetrize_union(int64_t n, const int32_t *counts, const int32_t *offsets
 int pg_fill_u32(uint32_t *p, int64_t n, uint32_t v, void *stream) {
     if (n < 0 || (n > 0 && !p)) return fail(1, "invalid argument");
     return check_hip(pg::fill_u32(p, n, v, as_stream(stream)), "fill");
+}
+
+int pg_noalloc_max_cells(int64_t *cells) {
+    if (!cells) return fail(1, "cells is null");
+    *cells = (int64_t)((kNoAllocScratchBytes - 256) / 12);
+    return 0;
+}
+
+int pg_compute_cells_noalloc(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *guesses,
+                             const int32_t *guess_offsets, int32_t guesses_sorted, int32_t guess_stride,
+                             const float *guess_d2, int32_t seed_window, const float *init_half_width, float *extent,
+                             int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts,
+                             int32_t *status, uint64_t *stats, int32_t waves, uint32_t *cell_cycles, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
+    if (begin == end) return 0;
+    const uint64_t need = pg::cell_scratch_bytes(end - begin);
+    if (need > kNoAllocScratchBytes)
+        return fail(1, "cell range too large for the static cell scratch (at most 1398080 cells per launch; "
+                       "split the range)");
+    void *ptr = nullptr;
+    if (int r = noalloc_scratch(&ptr)) return r;
+    return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, guess_stride,
+                               guess_d2, seed_window, init_half_width, extent, rows, rows_elems, row_offsets, counts,
+                               status, stats, ptr, kNoAllocScratchBytes, waves, cell_cycles, stream);
 }
 
 }  // extern "C"
