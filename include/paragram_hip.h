@@ -45,8 +45,11 @@ extern "C" {
  * largest tier: status 1 and an empty row), 13 rows that did not fit the row
  * pool, 14 (a maximum, not a sum) the pool elements such a launch needed,
  * 15 edge total E, 16 removal tests decided by expansion arithmetic,
- * 17..23 original ids (plus 1) of the first cells with such a test. */
-#define PG_NUM_STATS 24
+ * 17..23 original ids (plus 1) of the first cells with such a test,
+ * 24 cells the FP32-only tier gave up on at a removal test its filter could
+ * not decide, 25 ... at a new vertex of uncertain orientation (those cells
+ * are rebuilt by the FP64-capable twin), 26..31 reserved. */
+#define PG_NUM_STATS 32
 
 /* ABI version compiled into the library (== PG_ABI_VERSION). */
 int pg_abi_version(void);
@@ -252,6 +255,28 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
                            int32_t guess_stride, const float *guess_d2, int32_t *rows, uint64_t rows_elems,
                            int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
                            uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream);
+
+/* Writes the cloud offsets (host array, checked as for pg_build_batch_index)
+ * and the work quads of the whole range [0, n) into the batch workspace, and
+ * waits for the copies (a set-up call).  The two entry points below then read
+ * them from the workspace and launch kernels only, so a batch's index build,
+ * cells and CSR can be captured in one graph and replayed for new points of
+ * the same cloud sizes.  No counterpart in the reference. */
+int pg_batch_set_offsets(int64_t n, int32_t n_clouds, const int64_t *cloud_offsets, void *workspace,
+                         uint64_t workspace_bytes, void *stream);
+
+/* pg_build_batch_index with the cloud offsets already in the workspace
+ * (pg_batch_set_offsets). */
+int pg_build_batch_index_resident(int64_t n, int32_t n_clouds, const float *points, const float *weights,
+                                  int32_t leaf_size, void *workspace, uint64_t workspace_bytes, void *stream);
+
+/* pg_compute_cells_batch over all n cells with the offsets and work quads
+ * already in the workspace (pg_batch_set_offsets). */
+int pg_compute_cells_batch_resident(int64_t n, const void *workspace, int32_t n_clouds, const int32_t *guesses,
+                                    const int32_t *guess_offsets, int32_t guess_stride, const float *guess_d2,
+                                    int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts,
+                                    int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                                    int32_t waves, uint32_t *cell_cycles, void *stream);
 
 /* pg_knn over a batch index: each point's neighbours come from its own cloud
  * (the batch's ids are global); a point alone in its cloud gets its own id as

@@ -1,4 +1,4 @@
-"""Memory-admission planner for the per-cell row slab.
+"""Memory-admission planner for the per-cell row pool.
 
 Same environment semantics as the reference's plan_convex_cell_chunks
 (paragram/csrc/laguerre/chunking.h:16-56):
@@ -11,8 +11,9 @@ Same environment semantics as the reference's plan_convex_cell_chunks
   the result can be smaller than the forced count.
 
 The admission unit here is a block of 64 Morton-consecutive cells (one
-wavefront) and the chunked resource is the neighbour-row slab of those
-cells; cell scratch is per persistent wave and does not grow with N.
+wavefront) and the chunked resource is the row pool of those cells (32
+entries per cell by default), the larger of the two per-cell buffers (the
+cell scratch takes 12 bytes per cell).
 """
 from __future__ import annotations
 

@@ -21,11 +21,12 @@ import torch
 
 from ._chunking import plan_chunks
 
-NUM_STATS = 24
+NUM_STATS = 32
 STAT_NAMES = {
     0: "fp64_rechecks", 1: "unresolved", 2: "inconsistent", 3: "promoted", 4: "clips",
     5: "candidates", 6: "nodes", 7: "cells", 8: "asymmetric", 9: "dropped", 10: "leaves",
     11: "skipped", 12: "truncated", 13: "pool_short", 14: "pool_demand", 15: "edges", 16: "exact",
+    24: "bail_test", 25: "bail_vertex",
 }
 STAT_POOL_SHORT, STAT_POOL_DEMAND = 13, 14
 STAT_EXACT_LOG = 17  # 17..23: original ids + 1 of the first cells with an expansion-arithmetic test
@@ -58,6 +59,10 @@ class HipLibrary:
             "pg_exact_side": ([i64, vp, i64, vp, vp, vp], ctypes.c_int),
             "pg_exact_side_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_index_layout": ([i64, u64p], ctypes.c_int),
+            "pg_batch_set_offsets": ([i64, i32, vp, vp, u64, vp], ctypes.c_int),
+            "pg_build_batch_index_resident": ([i64, i32, vp, vp, i32, vp, u64, vp], ctypes.c_int),
+            "pg_compute_cells_batch_resident": ([i64, vp, i32, vp, vp, i32, vp, vp, u64, vp, vp, vp, vp, vp, u64,
+                                                 i32, vp, vp], ctypes.c_int),
             "pg_leaf_stats": ([i64, vp, vp, vp], ctypes.c_int),
             "pg_noalloc_max_cells": ([ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
             "pg_compute_cells_noalloc": ([i64, vp, i64, i64, vp, vp, i32, i32, vp, i32, vp, vp, vp, u64, vp, vp, vp,
@@ -122,7 +127,8 @@ class Instrument:
 def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, debug: bool = False,
                   initial_guesses=None, initial_guesses_offsets=None, leaf_size: int = -1,
                   instrument: Instrument | None = None, clouds=None, init_h=None, extent=None,
-                  radius_model=None, preclip: int | None = None, row_pool: int | None = None):
+                  radius_model=None, preclip: int | None = None, row_pool: int | None = None,
+                  waves_per_cu: int | None = None):
     """Returns ``(adjacency int32[E], offsets int32[N+1], status int32[N])`` on points' device.
 
     ``clouds``: optional host int64 array of cloud offsets (a batch of clouds,
@@ -136,6 +142,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
     clipped first when knn > 0 (default KNN_PRECLIP, 1..16).  ``row_pool``:
     initial row-pool elements per launch (default ``pg_row_pool_elems``);
     a pool that proves too small grows, so only the time depends on it.
+    ``waves_per_cu``: persistent tier-0 waves per compute unit (default: the
+    library's, 16); a launch choice like the others, never the result.
     """
     n = int(points.shape[0])
     if clouds is not None:
@@ -177,6 +185,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
         waves = ctypes.c_int32(0)
         lib.call("pg_default_waves", ctypes.byref(waves))
         waves = int(waves.value)
+        if waves_per_cu is not None:
+            waves = max(4, waves // 16 * int(waves_per_cu))
         stats = torch.zeros(NUM_STATS, dtype=torch.int64, device=dev)
         counts = torch.empty(n, **i32)
         status = torch.empty(n, **i32)
@@ -211,7 +221,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
                 raise ValueError("initial_guesses_offsets must be non-decreasing and index into initial_guesses")
 
         total_blocks = (n + CELL_BLOCK - 1) // CELL_BLOCK
-        full_bytes = 4 * lib.u64("pg_row_pool_elems", total_blocks * CELL_BLOCK)
+        full_bytes = (4 * lib.u64("pg_row_pool_elems", total_blocks * CELL_BLOCK)
+                      + lib.u64("pg_cell_scratch_bytes", n))
         free_bytes, _ = torch.cuda.mem_get_info(dev)
         plan = plan_chunks(total_blocks, CELL_BLOCK, full_bytes, free_bytes)
         chunk_cells = min(plan.chunk_threads, n)
@@ -313,7 +324,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             print("%s Construction time: %f ms" % (name, (time.perf_counter() - t1) * 1e3))
         if instrument is not None:
             instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
-            instrument.stats["exact_cells"] = [int(v) - 1 for v in st[STAT_EXACT_LOG:NUM_STATS].tolist() if v > 0]
+            instrument.stats["exact_cells"] = [int(v) - 1 for v in st[STAT_EXACT_LOG:STAT_EXACT_LOG + 7].tolist()
+                                               if v > 0]
             instrument.plan = plan
         return adjacency, offsets, status
 
@@ -357,9 +369,10 @@ def knn(lib: HipLibrary, points: torch.Tensor, k: int = 8, leaf_size: int = -1):
     return ids, d2
 
 
-def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds):
+def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds, cloud_entries=None):
     """Per-cloud ``(adjacency, offsets, status)`` of a batch diagram, with
-    cloud-local ids; views into two new device arrays, no host round trip."""
+    cloud-local ids; views into two new device arrays.  ``cloud_entries``
+    (host, optional): each cloud's adjacency entry count, else read once."""
     dev = offsets.device
     clouds = np.ascontiguousarray(clouds, dtype=np.int64)
     b = clouds.shape[0] - 1
@@ -385,7 +398,7 @@ def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Te
         o = local_off[int(firsts[c]):int(firsts[c]) + (z - a) + 1]
         out.append((o, status[a:z]))
     # adjacency slices need the per-cloud entry counts on the host (one read)
-    seg_host = seg.cpu().numpy()
+    seg_host = seg.cpu().numpy() if cloud_entries is None else np.asarray(cloud_entries)
     res = []
     for c in range(b):
         m = int(seg_host[c])

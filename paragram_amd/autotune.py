@@ -1,5 +1,6 @@
 """Autotuner for the launch choices that change speed but never the diagram:
-the LBVH leaf size and the number of nearest neighbours clipped first.
+the LBVH leaf size, the number of nearest neighbours clipped first, and the
+persistent waves per compute unit of the cell launch.
 
 ``tune(points, weights)`` times every candidate on a bounded prefix of the
 caller's own data (2 calls each after one warm-up) and returns the fastest;
@@ -25,6 +26,7 @@ from . import _api, _engine
 
 LEAF_SIZES = (8, 16, 32)
 PRECLIPS = (8, 16)
+WAVES_PER_CU = (12, 16, 24)  # 12 = what tier 0 keeps resident (3 per SIMD); more queue behind them
 SAMPLE = 200_000
 _cache: dict = {}
 
@@ -49,7 +51,9 @@ def _save_file():
 
 def tune(points: torch.Tensor, weights: torch.Tensor | None = None, clouds=None, sample: int = SAMPLE,
          force: bool = False) -> dict:
-    """The fastest {leaf_size, preclip} for data like ``points`` (timed on up to ``sample`` of them)."""
+    """The fastest {leaf_size, preclip, waves_per_cu} for data like ``points`` (timed on up to
+    ``sample`` of them): leaf size and pre-clip over their grid at the default waves, then the
+    waves per compute unit at the best of those."""
     from ._backend import _C
 
     n = int(points.shape[0])
@@ -69,19 +73,32 @@ def tune(points: torch.Tensor, weights: torch.Tensor | None = None, clouds=None,
         m = int(cl[-1])
         pts, w = pts[:m], None if w is None else w[:m]
     best, times = None, {}
+
+    def timed(cfg):
+        kw = dict(clouds=cl, preclip=cfg["preclip"], waves_per_cu=cfg["waves_per_cu"])
+        _engine.build_diagram(_C.lib, pts, w, 8, False, None, None, cfg["leaf_size"], **kw)
+        torch.cuda.synchronize(points.device)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            _engine.build_diagram(_C.lib, pts, w, 8, False, None, None, cfg["leaf_size"], **kw)
+        torch.cuda.synchronize(points.device)
+        t = (time.perf_counter() - t0) / 2
+        times[f"{cfg['leaf_size']}/{cfg['preclip']}/{cfg['waves_per_cu']}"] = round(t * 1e3, 3)
+        return t
+
     for leaf in LEAF_SIZES:
         for pre in PRECLIPS:
-            cfg = {"leaf_size": leaf, "preclip": pre}
-            _engine.build_diagram(_C.lib, pts, w, 8, False, None, None, leaf, clouds=cl, preclip=pre)
-            torch.cuda.synchronize(points.device)
-            t0 = time.perf_counter()
-            for _ in range(2):
-                _engine.build_diagram(_C.lib, pts, w, 8, False, None, None, leaf, clouds=cl, preclip=pre)
-            torch.cuda.synchronize(points.device)
-            t = (time.perf_counter() - t0) / 2
-            times[f"{leaf}/{pre}"] = round(t * 1e3, 3)
+            cfg = {"leaf_size": leaf, "preclip": pre, "waves_per_cu": 16}
+            t = timed(cfg)
             if best is None or t < best[0]:
                 best = (t, cfg)
+    for wpc in WAVES_PER_CU:
+        if wpc == 16:
+            continue
+        cfg = dict(best[1], waves_per_cu=wpc)
+        t = timed(cfg)
+        if t < best[0]:
+            best = (t, cfg)
     out = dict(best[1], sample=m, ms=times)
     _cache[key] = out
     _save_file()
@@ -95,7 +112,7 @@ def voronoi_diagram(points: torch.Tensor) -> _api.Diagram:
     points = _api._validate_points(points)
     cfg = tune(points)
     a, o, s = _engine.build_diagram(_C.lib, points, None, 8, False, None, None, cfg["leaf_size"],
-                                    preclip=cfg["preclip"])
+                                    preclip=cfg["preclip"], waves_per_cu=cfg.get("waves_per_cu"))
     return _api.Diagram(adjacency=a, offsets=o, status=s)
 
 
@@ -107,7 +124,7 @@ def power_diagram(points: torch.Tensor, weights: torch.Tensor) -> _api.Diagram:
     weights = _api._validate_weights(weights, points.shape[0], points.device)
     cfg = tune(points, weights)
     a, o, s = _engine.build_diagram(_C.lib, points, weights, 8, False, None, None, cfg["leaf_size"],
-                                    preclip=cfg["preclip"])
+                                    preclip=cfg["preclip"], waves_per_cu=cfg.get("waves_per_cu"))
     return _api.Diagram(adjacency=a, offsets=o, status=s)
 
 
@@ -119,5 +136,5 @@ def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets) -> list:
     off = _api._validate_cloud_offsets(cloud_offsets, points.shape[0])
     cfg = tune(points, None, off)
     a, o, s = _engine.build_diagram(_C.lib, points, None, 8, False, None, None, cfg["leaf_size"], clouds=off,
-                                    preclip=cfg["preclip"])
+                                    preclip=cfg["preclip"], waves_per_cu=cfg.get("waves_per_cu"))
     return [_api.Diagram(adjacency=x, offsets=y, status=z) for x, y, z in _engine.split_batch(a, o, s, off)]

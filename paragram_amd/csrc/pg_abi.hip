@@ -308,38 +308,22 @@ int pg_compute_cells_ex(int64_t n, const void *workspace, int64_t begin, int64_t
     return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells");
 }
 
-int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
-                           int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
-                           int32_t guess_stride, const float *guess_d2, int32_t *rows, uint64_t rows_elems,
-                           int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
-                           uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream) {
-    if (int r = check_n(n)) return r;
-    if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
-    if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
-    if (begin == end) return 0;
-    if (!workspace || !rows || !row_offsets || !counts || !status || !stats || !scratch)
-        return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
-    if (waves < 1) return fail(1, "waves must be positive");
+namespace {
+// Cells [begin, end) of a batch whose cloud offsets and work quads are in the
+// workspace already (written by the caller of this helper).
+int cells_batch(int64_t n, const void *workspace, int32_t n_clouds, int64_t begin, int64_t end,
+                const int32_t *guesses, const int32_t *guess_offsets, int32_t guess_stride, const float *guess_d2,
+                int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts, int32_t *status,
+                uint64_t *stats, void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles,
+                void *stream, const char *what) {
     const int w0 = (waves + 3) / 4 * 4;
-    if (scratch_bytes < pg::cell_scratch_bytes(end - begin)) return fail(1, "cell scratch too small");
     const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
-    char *ws = (char *)workspace;
-    std::vector<int64_t> quads((size_t)n_clouds + 1);
-    if (int r = pg_batch_quads(n_clouds, cloud_offsets, begin, end, quads.data())) return r;
-    // the copy is ordered before the kernels on the stream; hipMemcpyAsync
-    // returns once the pageable source has been staged
-    hipError_t e = hipMemcpyAsync(ws + B.quads, quads.data(), 8 * quads.size(), hipMemcpyHostToDevice,
-                                  as_stream(stream));
-    if (e != hipSuccess) return check_hip(e, "compute_cells_batch");
+    const char *ws = (const char *)workspace;
     pg::CellArgs a;
     a.n = n;
     a.index_ws = workspace;
     a.begin = begin;
     a.end = end;
-    if (guess_stride < 0 || guess_stride > 64) return fail(1, "guess_stride must be in [0, 64]");
-    if (guess_stride == 0 && (guesses == nullptr) != (guess_offsets == nullptr))
-        return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
-    if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
     a.guess_stride = guess_stride;
     a.guess_d2 = guess_stride > 0 ? guess_d2 : nullptr;
     a.guesses = guesses;
@@ -361,7 +345,98 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     a.cloud_cells = (const int64_t *)(ws + B.offsets);
     a.quads = (const int64_t *)(ws + B.quads);
     a.n_clouds = n_clouds;
-    return check_hip(pg::launch_cells(a, as_stream(stream)), "compute_cells_batch");
+    return check_hip(pg::launch_cells(a, as_stream(stream)), what);
+}
+
+int check_batch_cells(int64_t n, int64_t begin, int64_t end, const void *workspace, int32_t *rows,
+                      int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
+                      uint64_t scratch_bytes, const int32_t *guesses, const int32_t *guess_offsets,
+                      int32_t guess_stride, int32_t waves) {
+    if (begin < 0 || end > n || begin > end) return fail(1, "cell range out of bounds");
+    if (!workspace || !rows || !row_offsets || !counts || !status || !stats || !scratch)
+        return fail(1, "workspace, rows, row_offsets, counts, status, stats and scratch must be non-null");
+    if (waves < 1) return fail(1, "waves must be positive");
+    if (scratch_bytes < pg::cell_scratch_bytes(end - begin)) return fail(1, "cell scratch too small");
+    if (guess_stride < 0 || guess_stride > 64) return fail(1, "guess_stride must be in [0, 64]");
+    if (guess_stride == 0 && (guesses == nullptr) != (guess_offsets == nullptr))
+        return fail(1, "initial_guesses and initial_guesses_offsets must be provided together");
+    if (guess_stride > 0 && guesses == nullptr) return fail(1, "guess_stride needs guesses");
+    return 0;
+}
+}  // namespace
+
+int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,
+                           int64_t begin, int64_t end, const int32_t *guesses, const int32_t *guess_offsets,
+                           int32_t guess_stride, const float *guess_d2, int32_t *rows, uint64_t rows_elems,
+                           int64_t *row_offsets, int32_t *counts, int32_t *status, uint64_t *stats, void *scratch,
+                           uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
+    if (begin == end && begin >= 0 && end <= n) return 0;
+    if (int r = check_batch_cells(n, begin, end, workspace, rows, row_offsets, counts, status, stats, scratch,
+                                  scratch_bytes, guesses, guess_offsets, guess_stride, waves))
+        return r;
+    const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
+    std::vector<int64_t> quads((size_t)n_clouds + 1);
+    if (int r = pg_batch_quads(n_clouds, cloud_offsets, begin, end, quads.data())) return r;
+    // the copy is ordered before the kernels on the stream; hipMemcpyAsync
+    // returns once the pageable source has been staged
+    hipError_t e = hipMemcpyAsync((char *)workspace + B.quads, quads.data(), 8 * quads.size(), hipMemcpyHostToDevice,
+                                  as_stream(stream));
+    if (e != hipSuccess) return check_hip(e, "compute_cells_batch");
+    return cells_batch(n, workspace, n_clouds, begin, end, guesses, guess_offsets, guess_stride, guess_d2, rows,
+                       rows_elems, row_offsets, counts, status, stats, scratch, scratch_bytes, waves, cell_cycles,
+                       stream, "compute_cells_batch");
+}
+
+int pg_batch_set_offsets(int64_t n, int32_t n_clouds, const int64_t *cloud_offsets, void *workspace,
+                         uint64_t workspace_bytes, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (int r = check_cloud_offsets(n, n_clouds, cloud_offsets)) return r;
+    if (!workspace) return fail(1, "workspace is null");
+    const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
+    if (workspace_bytes < B.total) return fail(1, "batch workspace too small");
+    std::vector<int64_t> quads((size_t)n_clouds + 1);
+    if (int r = pg_batch_quads(n_clouds, cloud_offsets, 0, n, quads.data())) return r;
+    char *ws = (char *)workspace;
+    hipError_t e = hipMemcpyAsync(ws + B.offsets, cloud_offsets, 8 * ((size_t)n_clouds + 1), hipMemcpyHostToDevice,
+                                  as_stream(stream));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ws + B.quads, quads.data(), 8 * quads.size(), hipMemcpyHostToDevice, as_stream(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));  // a set-up call: the host arrays may go
+    return check_hip(e, "batch_set_offsets");
+}
+
+int pg_build_batch_index_resident(int64_t n, int32_t n_clouds, const float *points, const float *weights,
+                                  int32_t leaf_size, void *workspace, uint64_t workspace_bytes, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (n == 0) return 0;
+    if (!points || !workspace) return fail(1, "points and workspace must be non-null");
+    const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
+    if (workspace_bytes < B.total) return fail(1, "batch workspace too small");
+    int leaf = leaf_size < 0 ? pg::kDefaultLeaf : leaf_size;
+    if (leaf < 1) leaf = 1;
+    if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
+    const int64_t *dev_off = (const int64_t *)((char *)workspace + B.offsets);
+    return check_hip(pg::build_batch_index(n, n_clouds, points, weights, dev_off, leaf, workspace, as_stream(stream)),
+                     "build_batch_index_resident");
+}
+
+int pg_compute_cells_batch_resident(int64_t n, const void *workspace, int32_t n_clouds, const int32_t *guesses,
+                                    const int32_t *guess_offsets, int32_t guess_stride, const float *guess_d2,
+                                    int32_t *rows, uint64_t rows_elems, int64_t *row_offsets, int32_t *counts,
+                                    int32_t *status, uint64_t *stats, void *scratch, uint64_t scratch_bytes,
+                                    int32_t waves, uint32_t *cell_cycles, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (n == 0) return 0;
+    if (int r = check_batch_cells(n, 0, n, workspace, rows, row_offsets, counts, status, stats, scratch,
+                                  scratch_bytes, guesses, guess_offsets, guess_stride, waves))
+        return r;
+    return cells_batch(n, workspace, n_clouds, 0, n, guesses, guess_offsets, guess_stride, guess_d2, rows, rows_elems,
+                       row_offsets, counts, status, stats, scratch, scratch_bytes, waves, cell_cycles, stream,
+                       "compute_cells_batch_resident");
 }
 
 int pg_exact_side(int64_t n, const void *workspace, int64_t m, const int32_t *queries, int32_t *out, void *stream) {

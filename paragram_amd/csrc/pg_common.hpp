@@ -55,10 +55,18 @@ enum StatSlot : int {
     kStatEdges = 15,         // edge total E (written by the symmetry check)
     kStatExact = 16,         // removal tests decided by expansion arithmetic (FP64 undecided)
     kStatExactLog = 17,      // 17..23: original ids of the first cells with such a test, plus 1
-    kNumStats = 24,
+    kStatBailTest = 24,      // FP32-only tier 0 gave a cell up: a removal test its filter could not decide
+    kStatBailVertex = 25,    // ... a new vertex whose FP32 orientation was not safely positive
+    kNumStats = 32,
 };
 constexpr int kExactLogSlots = 7;
-constexpr int kNumCtr = 12;  // statistics slots 0..11 are first summed per group in LDS
+// Statistics first summed per group in LDS: slots 0..11, then the two bail
+// counters (LDS slots 12 and 13).
+constexpr int kCtrBailTest = 12, kCtrBailVertex = 13;
+constexpr int kNumCtr = 14;
+__host__ __device__ constexpr int ctr_stat_slot(int k) {
+    return k == kCtrBailTest ? kStatBailTest : (k == kCtrBailVertex ? kStatBailVertex : k);
+}
 
 // ------------------------------------------------------------------ LBVH
 // Internal node of the Karras hierarchy in traversal form: the boxes of both

@@ -730,6 +730,7 @@ struct Group {
             }
         } else if (gany(unsure != 0u)) {
             overflow = bail = true;
+            if (gl == 0) count(kCtrBailTest);
             return false;
         }
         if (!gany(mine != 0u)) return false;  // no vertex beyond: the plane does not cut
@@ -857,6 +858,7 @@ struct Group {
                 fix_vertices(added);
             } else {
                 overflow = bail = true;
+                if (gl == 0) count(kCtrBailVertex);
                 return false;
             }
         }
@@ -1331,7 +1333,7 @@ struct Group {
 #pragma unroll
         for (int k = 0; k < kNumCtr; ++k) {
             const uint32_t s = wave_sum(vals[k]);
-            if (lane64 == 0 && s) atomicAdd(&prm.stats[k], (unsigned long long)s);
+            if (lane64 == 0 && s) atomicAdd(&prm.stats[ctr_stat_slot(k)], (unsigned long long)s);
         }
     }
 };

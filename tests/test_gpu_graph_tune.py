@@ -45,6 +45,37 @@ def test_graph_replays_equal_eager_calls(gpu, weighted):
             assert torch.equal(x, y), f"replay for seed {seed} differs"
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_graphed_batch_replays_equal_eager_batches(gpu, weighted):
+    """A batch of clouds captured in one graph (index, kNN pre-clip, cells,
+    CSR): replays for new coordinates of the same cloud sizes, with eager
+    batched calls between them, equal the eager batched calls."""
+    import torch
+
+    import paragram_amd
+    from paragram_amd.graph import GraphedBatch
+
+    log(f"graphed batch, weighted={weighted}")
+    sizes = [3000, 1, 0, 17, 5000, 64, 2]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    g = GraphedBatch(off, weighted=weighted, device=gpu)
+    for seed in (4, 5, 4):
+        rng = np.random.default_rng(seed)
+        pts = rng.random((n, 3)).astype(np.float32)
+        w = rng.uniform(0, 1e-4, n).astype(np.float32) if weighted else None
+        P = torch.from_numpy(pts).to(gpu)
+        W = None if w is None else torch.from_numpy(w).to(gpu)
+        want = (paragram_amd.batched_voronoi_diagram(P, off) if w is None
+                else paragram_amd.batched_power_diagram(P, W, off))
+        got = g(P, W)
+        torch.cuda.synchronize()
+        assert len(got) == len(want) == len(sizes)
+        for c, ((a, o, s), d) in enumerate(zip(got, want)):
+            assert torch.equal(a, d.adjacency) and torch.equal(o, d.offsets) and torch.equal(s, d.status), \
+                f"replay for seed {seed}: cloud {c} differs"
+
+
 def test_autotuned_calls_equal_plain(gpu):
     import torch
 
@@ -57,6 +88,7 @@ def test_autotuned_calls_equal_plain(gpu):
     cfg = autotune.tune(P, force=True)
     log(f"tuned: {cfg}")
     assert cfg["leaf_size"] in autotune.LEAF_SIZES and cfg["preclip"] in autotune.PRECLIPS
+    assert cfg["waves_per_cu"] in autotune.WAVES_PER_CU
     a = autotune.voronoi_diagram(P)
     b = paragram_amd.voronoi_diagram(P)
     assert torch.equal(a.adjacency, b.adjacency) and torch.equal(a.offsets, b.offsets)
