@@ -10,10 +10,19 @@ Prints one JSON line.  ``value`` is Mcells/s = N * steps / timed seconds / 1e6.
 algorithmic bytes per cell B = 24 + 20 F (F = E / N, measured): the seed's
 float4, each true neighbour's float4 once, the neighbour id written, plus
 offset and status; achieved = N B / (mean cell-kernel duration from HIP
-events on the launch stream), peak 8 TB/s.  ``cpu_baseline`` times the CPU
-oracle (FP64 filter + exact fallback, OpenMP) on a seeded sample of cells of
-the same cloud, and the same sample gives ``exact_cells``: the fraction of
-sampled GPU rows and statuses equal to the oracle's.
+events on the launch stream), peak 8 TB/s.  ``cpu_baseline`` times the same
+algorithm on the host cores (paragram_amd.cpu, C++/OpenMP, all cores of the
+process's affinity set) over the whole cloud; ``exact_cells`` compares a
+seeded sample of GPU rows and statuses with the CPU oracle (FP64 filter +
+exact fallback, k-d tree).
+
+``extra`` (skipped with --no-extra) measures the rest of the metric in the
+same run: the 10M-point power configuration in the plain form and in the
+learned-radius form (the committed network), their ratio, the 256-cloud
+LiDAR batch in one launch, and the 10k and 2M configurations; each entry has
+its ms per call (median of the timed calls), Mcells/s, FP64-recheck and
+exact-arithmetic counts, and an oracle sample (every cell up to 1M points, a
+seeded 100k-cell sample at 10M, 200 cells of each of 32 clouds of the batch).
 """
 from __future__ import annotations
 
@@ -104,6 +113,116 @@ def summarize_pmc(fetch_dir: str, write_dir: str, copy_mib: int, workload_key: s
     return rec
 
 
+STAT_KEYS = ("fp64_rechecks", "exact", "unresolved", "promoted", "truncated", "pool_short", "edges")
+
+
+def _time_calls(fn, warmup: int, steps: int):
+    """Median and all timings (ms) of steps calls after warmup calls, each
+    bracketed by device synchronisations."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ms = []
+    res = None
+    for _ in range(steps):
+        t = time.perf_counter()
+        res = fn()
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t) * 1e3)
+    return float(sorted(ms)[len(ms) // 2]), [round(x, 3) for x in ms], res
+
+
+def _sample_check(adj, off, status, pts_np, w_np, sample):
+    import numpy as np
+
+    import oracle
+
+    ref = oracle.cells(pts_np, w_np, sample)
+    ok = 0
+    for k, i in enumerate(sample):
+        ok += int(status[i] == ref.status[k] and np.array_equal(adj[off[i]:off[i + 1]], ref.row(k)))
+    return ok
+
+
+def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
+    """The other configurations of the metric, timed in this run (see the module docstring)."""
+    import numpy as np
+    import torch
+
+    from paragram_amd import _engine, learned, workloads
+
+    res = {}
+
+    def entry(desc, n, ms, all_ms, instr, exact):
+        st = instr.stats or {}
+        e = {"workload": desc, "n_points": n, "ms_per_call": round(ms, 3), "calls_ms": all_ms,
+             "mcells_s": round(n / ms / 1e3, 3), "stats": {k: st.get(k) for k in STAT_KEYS}}
+        if exact is not None:
+            e["exact_cells"] = exact
+        return e
+
+    def single(key, gen_args, weights_key=None, steps=3, model=None, sample_count=None):
+        gen, n, seed, desc = WORKLOADS[key]
+        made = getattr(workloads, gen)(*gen_args)
+        pts_np, w_np = made if isinstance(made, tuple) else (made, None)
+        pts = torch.from_numpy(pts_np).to(dev)
+        w = None if w_np is None else torch.from_numpy(w_np).to(dev).reshape(-1, 1)
+        instr = _engine.Instrument()
+
+        def call():
+            instr.cell_events.clear()
+            return _engine.build_diagram(lib, pts, w, 8, False, None, None, -1, instrument=instr, radius_model=model)
+
+        ms, all_ms, out = _time_calls(call, 1, steps)
+        exact = None
+        if cpu_check:
+            adj, off, status = (t.cpu().numpy() for t in out)
+            sample = workloads.oracle_sample(pts_np.shape[0], seed, sample_count or pts_np.shape[0])
+            ok = _sample_check(adj, off, status, pts_np, w_np, sample)
+            exact = {"checked": int(sample.shape[0]), "exact": ok, "frac": ok / sample.shape[0]}
+        del out, pts, w
+        torch.cuda.empty_cache()
+        return desc.format(n=n, seed=seed, total=pts_np.shape[0]), pts_np.shape[0], ms, all_ms, instr, exact
+
+    res["c1"] = entry(*single("c1", (10_000, 0), steps=5))
+    res["c3"] = entry(*single("c3", (2_000_000, 2), steps=3, sample_count=20_000))
+    model = learned.load()
+    plain = entry(*single("c5", (10_000_000, 4), steps=3, sample_count=100_000))
+    desc, n, ms, all_ms, instr, exact = single("c5", (10_000_000, 4), steps=3, model=model, sample_count=100_000)
+    lrn = entry(desc.replace("(plain form)", "(learned-radius form: committed radius_mlp.npz)"), n, ms, all_ms, instr,
+                exact)
+    res["c5"] = {"plain": plain, "learned": lrn,
+                 "learned_over_plain": round(plain["ms_per_call"] / lrn["ms_per_call"], 4)}
+
+    # the LiDAR batch: one launch for all 256 clouds
+    gen, n_clouds, seed, desc = WORKLOADS["c4"]
+    cl = workloads.lidar_batch(seed, n_clouds=n_clouds)
+    clouds = np.concatenate([[0], np.cumsum([c.shape[0] for c in cl])]).astype(np.int64)
+    pts = torch.from_numpy(np.concatenate(cl)).to(dev)
+    instr = _engine.Instrument()
+
+    def batch_call():
+        instr.cell_events.clear()
+        out = _engine.build_diagram(lib, pts, None, 8, False, None, None, -1, instrument=instr, clouds=clouds)
+        return _engine.split_batch(*out, clouds)
+
+    ms, all_ms, parts = _time_calls(batch_call, 1, 3)
+    exact = None
+    if cpu_check:
+        ok = checked = 0
+        for c in range(0, n_clouds, n_clouds // 32):
+            a, o, s = (t.cpu().numpy() for t in parts[c])
+            sample = workloads.oracle_sample(cl[c].shape[0], seed + c, 200)
+            ok += _sample_check(a, o, s, cl[c], None, sample)
+            checked += sample.shape[0]
+        exact = {"checked": checked, "exact": ok, "frac": ok / checked}
+    res["c4"] = entry(desc.format(n=n_clouds, seed=seed, total=int(clouds[-1])), int(clouds[-1]), ms, all_ms, instr,
+                      exact)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +234,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=None, help="override the workload's seed")
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="cells timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--form", choices=("plain", "learned"), default="plain",
                     help="learned: start each cell from the committed radius network's prediction")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -219,28 +339,39 @@ def main() -> int:
     exact = None
     if not args.no_cpu:
         import oracle
+        from paragram_amd import cpu as host
 
         cores = len(os.sched_getaffinity(0))
-        if clouds is not None:  # the oracle on cells of the first cloud
+        if clouds is not None:  # one cloud of the batch on the host, and the oracle on it
             pts_np, w_np = pts_np[clouds[0]:clouds[1]], None
+        # the same clipping algorithm on the host cores, over the whole cloud
+        hl = host.CpuLibrary()
+        hl.diagram(pts_np[:2000], None if w_np is None else w_np[:2000], nthreads=cores)  # thread start-up
+        th = time.perf_counter()
+        h_adj, h_off, h_st, h_stats = hl.diagram(pts_np, w_np, nthreads=cores)
+        t_host = time.perf_counter() - th
+        same = None
+        if clouds is None:
+            same = bool(np.array_equal(h_adj, adj) and np.array_equal(h_off, off) and np.array_equal(h_st, status))
+        cpu = {"value": round(pts_np.shape[0] / t_host / 1e6, 4), "unit": "Mcells/s", "cores": cores, "kind": "port",
+               "sample": f"the whole {pts_np.shape[0]}-point cloud (index, cells and CSR, {t_host:.2f} s) with the "
+                         "GPU kernels' algorithm on the host cores (paragram_amd/csrc/pg_cpu.cpp: filtered FP32, FP64 "
+                         "recheck, exact fallback, kNN pre-clip, same prunes), C++/OpenMP; "
+                         + ("output identical to the GPU's" if same else
+                            "output differs from the GPU's" if same is not None else "cloud 0 of the batch")}
+        # the oracle (FP64 filter + exact fallback, k-d tree) as the checker on a seeded sample
         sample = workloads.oracle_sample(pts_np.shape[0], args.seed, args.cpu_sample)
-        # the oracle builds its k-d tree per call: time that separately
-        tb = time.perf_counter()
-        oracle.cells(pts_np, w_np, sample[:1], nthreads=cores)
-        t_build = time.perf_counter() - tb
-        tc = time.perf_counter()
         ref = oracle.cells(pts_np, w_np, sample, nthreads=cores)
-        t_cells = time.perf_counter() - tc - t_build
-        cpu = {"value": round(sample.shape[0] / max(t_cells, 1e-9) / 1e6, 4), "unit": "Mcells/s",
-               "cores": cores, "kind": "port",
-               "sample": f"{sample.shape[0]} seeded cells of the same {pts_np.shape[0]}-point cloud; CPU oracle "
-                         "(FP64 filtered predicates, Shewchuk-expansion exact fallback), OpenMP; k-d tree build "
-                         f"({t_build:.2f}s) excluded"}
         ok = 0
         for k, i in enumerate(sample):  # (cloud 0 of a batch: its ids are the global ids)
             g = adj[off[i]:off[i + 1]]
             ok += int(status[i] == ref.status[k] and np.array_equal(g, ref.row(k)))
         exact = {"checked": int(sample.shape[0]), "exact": ok, "frac": ok / sample.shape[0]}
+
+    extra = None
+    if not args.no_extra and args.workload == "c2" and args.n == n_def and args.seed == seed_def:
+        del out, adj, off, status
+        extra = measure_extra(lib, dev, cpu_check=not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -268,6 +399,7 @@ def main() -> int:
         "exact_cells": exact,
         "per_cloud_calls_ms": None if per_cloud_ms is None else round(per_cloud_ms, 3),
         "stats": instr.stats,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
     return 0
