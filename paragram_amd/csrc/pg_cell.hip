@@ -56,6 +56,7 @@ struct CellParams {
     int seed_window;              // Morton neighbours offered first (0: none)
     int32_t *rows;                // row pool (every tier bump-allocates its rows)
     uint64_t rows_elems;          // pool capacity
+    uint32_t pool_chunk;          // pool entries a tier-0 wave takes at a time (0: exactly what a round needs)
     int64_t *row_offsets;         // [n] sorted cell -> row start in `rows`
     int32_t *counts;
     int32_t *status;
@@ -210,6 +211,13 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;
     const int64_t blocks = max_blocks < blocks_needed ? max_blocks : blocks_needed;
+    // A tier-0 wave takes the pool in chunks, one device-wide atomic per
+    // several rounds; what the waves leave unused is at most 1/16 of the pool.
+    {
+        const uint64_t waves0 = (uint64_t)(blocks > 0 ? blocks : 1) * (kT0Threads / kWave);
+        const uint64_t per_wave = prm.rows_elems / (16 * waves0);
+        prm.pool_chunk = per_wave >= 64 ? (uint32_t)(per_wave < 1024 ? per_wave : 1024) : 0u;
+    }
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
     if (hipError_t e = fill_u32(prm.exact_list, count, 0u, stream)) return e;
     hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream, prm);

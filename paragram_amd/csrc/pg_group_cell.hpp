@@ -356,7 +356,10 @@ struct Group {
     bool bounded;
     int status;
     bool overflow;  // the cell left this tier (capacity, or `bail`)
-    bool bail;      // an FP32 test was undecided and the tier has no FP64 path
+    // why an FP32-only tier gave the cell up (0: it did not): 1 an undecided
+    // removal test, 2 an undecided vertex orientation, 3 a predicted cube
+    // that cut the cell (counted per cell after the traversal)
+    uint32_t bail;
     bool cube;      // the cell started inside a predicted cube (see init_box)
     bool active;  // the group holds a cell in this round
     bool dirty;   // the cell changed since the AABB was computed
@@ -729,8 +732,8 @@ struct Group {
                 if (recheck64(gl + G * k, q.owner)) mine |= 1u << k;
             }
         } else if (gany(unsure != 0u)) {
-            overflow = bail = true;
-            if (gl == 0) count(kCtrBailTest);
+            overflow = true;
+            bail = 1u;
             return false;
         }
         if (!gany(mine != 0u)) return false;  // no vertex beyond: the plane does not cut
@@ -857,8 +860,8 @@ struct Group {
             if constexpr (kExact) {
                 fix_vertices(added);
             } else {
-                overflow = bail = true;
-                if (gl == 0) count(kCtrBailVertex);
+                overflow = true;
+                bail = 2u;
                 return false;
             }
         }
@@ -1259,7 +1262,7 @@ struct Group {
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
         status = kSuccess;
         overflow = false;
-        bail = false;
+        bail = 0u;
         const uint32_t orig = perm[cell];
         init_box((!kExact && prm.init_h) ? 2.0f * prm.init_h[orig] : 0.0f);
         in_guess = true;
@@ -1420,7 +1423,14 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     __shared__ GroupLDS<kT0Verts, kT0Planes> lds[kT0Threads / kT0Lanes];
     __shared__ CandLDS<kT0Lanes> cands[kT0Threads / kT0Lanes];
     __shared__ CellParams sprm;
+    // each wave's unused part of the row pool: next entry and entries left
+    __shared__ unsigned long long chunk_at[kT0Threads / kWave];
+    __shared__ uint32_t chunk_left[kT0Threads / kWave];
     if (threadIdx.x == 0) sprm = prm;
+    if (threadIdx.x < kT0Threads / kWave) {
+        chunk_at[threadIdx.x] = 0ull;
+        chunk_left[threadIdx.x] = 0u;
+    }
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int grp = lane / kT0Lanes;
@@ -1443,13 +1453,29 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         if (kExactTier && active && g.gl == 0 && g.S->exact_events) log_exact(P, orig, g.S->exact_events);
         // a cell built inside a predicted cube stands only if the cube cut nothing
         if (active && !kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
-            g.overflow = g.bail = true;
+            g.overflow = true, g.bail = 3u;
         const bool emit = active && !g.overflow;
         const int F = emit ? g.stage_row(P) : 0;
-        // one row-pool allocation for the wave's four rows
+        // the wave's four rows, back to back in the wave's current pool chunk
         const int f0 = rl(F, 0), f1 = rl(F, 16), f2 = rl(F, 32), f3 = rl(F, 48);
-        const uint64_t base = pool_take(P, (uint32_t)(f0 + f1 + f2 + f3)) + (uint64_t)(grp > 0 ? f0 : 0) +
-                              (uint64_t)(grp > 1 ? f1 : 0) + (uint64_t)(grp > 2 ? f2 : 0);
+        // (the chunk's position lives in LDS, not in registers across the traversal)
+        const uint32_t need = (uint32_t)(f0 + f1 + f2 + f3);
+        const int wv = threadIdx.x / kWave;
+        uint32_t left = uniform(chunk_left[wv]);
+        uint64_t at;
+        if (need > left) {
+            left = need > P.pool_chunk ? need : P.pool_chunk;
+            at = pool_take(P, left);
+        } else {
+            const unsigned long long a = chunk_at[wv];
+            at = ((uint64_t)uniform((uint32_t)(a >> 32)) << 32) | (uint64_t)uniform((uint32_t)a);
+        }
+        const uint64_t base = at + (uint64_t)(grp > 0 ? f0 : 0) + (uint64_t)(grp > 1 ? f1 : 0) +
+                              (uint64_t)(grp > 2 ? f2 : 0);
+        if (lane == 0) {
+            chunk_at[wv] = at + need;
+            chunk_left[wv] = left - need;
+        }
         if (emit) {
             if (base + (uint64_t)F <= P.rows_elems) {
                 if (g.gl == 0) P.row_offsets[cell] = (int64_t)base;
@@ -1459,6 +1485,7 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
             }
         } else if (active && g.gl == 0) {
             if (!kExactTier && g.bail) {  // flag for the FP64-capable instance
+                if (g.bail < 3u) g.count(g.bail == 1u ? kCtrBailTest : kCtrBailVertex);
                 P.exact_list[c] = 1u;
                 atomicAdd(&P.counters[5], 1u);
             } else {  // queue for the large-capacity tier
