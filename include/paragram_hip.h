@@ -307,6 +307,17 @@ int pg_exact_side(int64_t n, const void *workspace, int64_t m, const int32_t *qu
 int pg_exact_side_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
                        const float *box_hi, int32_t *out);
 
+/* The exact vertex position the FP64-capable cell tiers fall back to where
+ * the FP64 bound on a vertex is loose (three nearly coaxial planes), on the
+ * host for one vertex: seed and box as pg_exact_side_host, planes 3 x
+ * (x, y, z, w), box_codes[3]; out[0..2] = the vertex y = 2 (x - seed) of the
+ * three planes rounded to FP32, out[3] = a bound on the rounding error (max
+ * norm), or -1 when the orientation D is not positive (no vertex).  The
+ * reference computes vertices in FP32 only (voronoi_convex_cell.cuh:226-254);
+ * this is for checking the device code against rational arithmetic. */
+int pg_exact_vertex_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
+                         const float *box_hi, float *out);
+
 /* Bytes of temporary storage for pg_scan_counts over n values. */
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes);
 

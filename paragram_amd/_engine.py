@@ -58,6 +58,7 @@ class HipLibrary:
             "pg_scan_tmp_bytes": ([i64, u64p], ctypes.c_int),
             "pg_exact_side": ([i64, vp, i64, vp, vp, vp], ctypes.c_int),
             "pg_exact_side_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_exact_vertex_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_index_layout": ([i64, u64p], ctypes.c_int),
             "pg_batch_set_offsets": ([i64, i32, vp, vp, u64, vp], ctypes.c_int),
             "pg_build_batch_index_resident": ([i64, i32, vp, vp, i32, vp, u64, vp], ctypes.c_int),

@@ -464,6 +464,24 @@ int pg_exact_side_host(const float *seed, const float *planes, const int32_t *bo
     return 0;
 }
 
+int pg_exact_vertex_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
+                         const float *box_hi, float *out) {
+    if (!seed || !planes || !box_codes || !box_lo || !box_hi || !out) return fail(1, "invalid argument");
+    pg::xp::PlaneIn pl[3];
+    for (int e = 0; e < 3; ++e) {
+        if (box_codes[e] < -1 || box_codes[e] > 5) return fail(1, "box codes must be -1 (point) or 0..5");
+        pl[e] = pg::xp::PlaneIn{planes[4 * e], planes[4 * e + 1], planes[4 * e + 2], planes[4 * e + 3], box_codes[e]};
+    }
+    float y[3], w;
+    if (!pg::xp::exact_position(pl, seed, box_lo, box_hi, y, w)) {
+        out[0] = out[1] = out[2] = 0.0f;
+        out[3] = -1.0f;
+        return 0;
+    }
+    out[0] = y[0], out[1] = y[1], out[2] = y[2], out[3] = w;
+    return 0;
+}
+
 int pg_scan_tmp_bytes(int64_t n, uint64_t *bytes) {
     if (n < 0 || !bytes) return fail(1, "invalid argument");
     *bytes = 4 * (uint64_t)pg::scan_tmp_elems(n);

@@ -373,21 +373,47 @@ class Cell {
             const float ex = (kG32 * pNx + std::fabs(x) * eD) * den + std::fabs(x) * kSlack;
             const float ey = (kG32 * pNy + std::fabs(y) * eD) * den + std::fabs(y) * kSlack;
             const float ez = (kG32 * pNz + std::fabs(z) * eD) * den + std::fabs(z) * kSlack;
-            return store(x, y, z, std::max(ex, std::max(ey, ez)) + kTiny32, t);
+            const float w = std::max(ex, std::max(ey, ez)) + kTiny32;
+            // a radius beyond 2^-12 of the position (nearly parallel or
+            // coaxial planes) goes to FP64, as in the FP64-capable GPU tiers
+            if (w <= 0x1p-12f * std::max(std::fabs(x), std::max(std::fabs(y), std::fabs(z))))
+                return store(x, y, z, w, t);
         }
-        // FP64 position, or a radius of +inf when FP64 cannot bound it either
+        // FP64 position; where its bound is loose (more than 2^-12 of the
+        // position) the exact one; a radius of +inf when neither bounds it
         double N[3], DD, pN[3], pDD;
         fp64_vertex(t, N, DD, pN, pDD);
         const double e64 = kG64 * pDD;
-        if (!(DD > 2.0 * e64 && pDD > kTiny64)) return store(0.f, 0.f, 0.f, INFINITY, t);
-        float xs[3], eps = 0.0f;
-        for (int m = 0; m < 3; ++m) {
-            const double X = N[m] / DD;
-            const double E = (kG64 * pN[m] + std::fabs(X) * e64) / (DD - e64) * (1.0 + 1e-12) + std::fabs(X) * 1e-12;
-            xs[m] = (float)X;
-            eps = std::max(eps, (float)E + std::fabs(xs[m]) * kSlack);
+        float xs[3] = {0.f, 0.f, 0.f}, eps = INFINITY;
+        if (DD > 2.0 * e64 && pDD > kTiny64) {
+            float ym = 0.0f;
+            eps = 0.0f;
+            for (int m = 0; m < 3; ++m) {
+                const double X = N[m] / DD;
+                const double E =
+                    (kG64 * pN[m] + std::fabs(X) * e64) / (DD - e64) * (1.0 + 1e-12) + std::fabs(X) * 1e-12;
+                xs[m] = (float)X;
+                eps = std::max(eps, (float)E + std::fabs(xs[m]) * kSlack);
+                ym = std::max(ym, std::fabs(xs[m]));
+            }
+            eps += kTiny32;
+            if (eps <= ym * 0x1p-12f) return store(xs[0], xs[1], xs[2], eps, t);
         }
-        return store(xs[0], xs[1], xs[2], eps + kTiny32, t);
+        pg::xp::PlaneIn in[3];
+        for (int e = 0; e < 3; ++e) {
+            const int o = pl[t[e]].owner;
+            if (o >= 0) {
+                const P4 &p = I->pts[o];
+                in[e] = pg::xp::PlaneIn{p.x, p.y, p.z, p.w, -1};
+            } else {
+                in[e] = pg::xp::PlaneIn{0.f, 0.f, 0.f, 0.f, -1 - o};
+            }
+        }
+        const float s4[4] = {sx, sy, sz, sw};
+        float ye[3], we;
+        if (pg::xp::exact_position(in, s4, I->box_lo, I->box_hi, ye, we) && we + kTiny32 < eps)
+            return store(ye[0], ye[1], ye[2], we + kTiny32, t);
+        return store(xs[0], xs[1], xs[2], eps, t);
     }
 
     // vertex v strictly beyond plane q?  FP32 filter, FP64 recheck, exact

@@ -294,5 +294,62 @@ __host__ __device__ inline bool exact_signs(const PlaneIn pl[4], const float s[4
     return true;
 }
 
+// The value of an expansion to within a few units in the last place of a
+// double: its components summed smallest first (they do not overlap, so each
+// partial sum at least doubles the last and the rounding errors stay below
+// one unit in the last place of the result).
+__host__ __device__ inline double approx(const Ex &e) {
+#pragma clang fp contract(off)
+    double v = 0.0;
+    for (int i = 0; i < e.n; ++i) v += e.c[i];
+    return v;
+}
+
+// Position of the vertex of planes pl[0..2] of the cell of seed s with box
+// [lo, hi]: y = N / D, N and D as in exact_signs and held exactly, so that
+// y is known to a few units in the last place of a double however small D
+// is (three nearly coaxial planes, where the FP64 bound on N / D fails).
+// out[k] = y_k rounded to FP32 and w >= max_k |y_k - out[k]|.  Returns false
+// when D is not positive or an expansion exceeds its capacity.
+__host__ __device__ inline bool exact_position(const PlaneIn pl[3], const float s[4], const float lo[3],
+                                               const float hi[3], float out[3], float &w) {
+    E2 a[3][3];
+    for (int t = 0; t < 3; ++t) plane_a(a[t], pl[t], s);
+    Ex D;
+    det3(D, a[0], a[1], a[2]);
+    if (D.n < 0 || sign(D) <= 0) return false;
+    Ex H[3];
+    for (int t = 0; t < 3; ++t) plane_H(H[t], a[t], pl[t], s, lo, hi);
+    const double d = approx(D);
+    double err = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const int i = (k + 1) % 3, j = (k + 2) % 3;
+        Ex acc, x, y, p, q, r;
+        set_zero(acc);
+        for (int t = 0; t < 3; ++t) {
+            const E2 *u = a[(t + 1) % 3], *v = a[(t + 2) % 3];
+            from2(x, u[i]);
+            from2(y, v[j]);
+            mul(p, x, y);
+            from2(x, u[j]);
+            from2(y, v[i]);
+            mul(q, x, y);
+            add(r, p, q, -1.0);  // (u x v)_k
+            mul(p, H[t], r);
+            add(q, acc, p);
+            acc = q;
+        }
+        if (acc.n < 0) return false;
+        // N_k / D from two values each within 2^-52 relative, one division:
+        // within 1e-15 relative; 1e-13 of |y_k| covers it with room
+        const double yk = approx(acc) / d;
+        out[k] = (float)yk;
+        const double e = fabs(yk - (double)out[k]) + fabs(yk) * 1e-13;
+        err = e > err ? e : err;
+    }
+    w = (float)(err * (1.0 + 1e-6)) + 1e-30f;
+    return true;
+}
+
 }  // namespace xp
 }  // namespace pg

@@ -234,25 +234,59 @@ __device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, c
     return flags | 2 | ((Pv > 0.0) == (D >= 0.0) ? 1 : 0);
 }
 
+// Position and error radius of a vertex of planes t by expansion
+// arithmetic (pg_exact.hpp); false when its orientation is not positive or
+// the expansions overflow.  Its own call site keeps the expansions' private
+// arrays off the common path.
+template <int kB>
+__device__ __noinline__ bool exact_vertex(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+                                          float sx, float sy, float sz, float sw, float4 &out) {
+    xp::PlaneIn pl[3];
+    for (int e = 0; e < 3; ++e) {
+        const int o = __float_as_int(plx[tri_get<kB>(t, e)].y);
+        if (o >= 0) {
+            const float4 p = pts[o];
+            pl[e] = xp::PlaneIn{p.x, p.y, p.z, p.w, -1};
+        } else {
+            pl[e] = xp::PlaneIn{0.f, 0.f, 0.f, 0.f, -1 - o};
+        }
+    }
+    const float s4[4] = {sx, sy, sz, sw};
+    float y[3], w;
+    if (!xp::exact_position(pl, s4, hdr->box_lo, hdr->box_hi, y, w)) return false;
+    out = make_float4(y[0], y[1], y[2], w + kTiny32);
+    return true;
+}
+
 // Position and error radius of a vertex whose FP32 orientation is not
-// safely positive; the radius is +inf when FP64 cannot bound it either
-// (every test on that vertex then goes to FP64).
+// safely positive (or, in the tiers with an FP64 path, whose FP32 radius
+// would be large).  FP64 first; where its bound is loose (more than 2^-12 of
+// the position: D within a few thousand rounding errors of 0) the position
+// is computed exactly; the radius is +inf when neither can bound it (every
+// test on that vertex then goes to FP64).
 template <int kB>
 __device__ __noinline__ float4 fp64_position(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
                                              float sx, float sy, float sz, float sw) {
     double N[3], D, pN[3], pD;
     fp64_vertex<kB>(plx, pts, hdr, t, sx, sy, sz, sw, N, D, pN, pD);
     const double e64 = kG64 * pD;
-    if (!(D > 2.0 * e64 && pD > kTiny64)) return make_float4(0.f, 0.f, 0.f, __int_as_float(0x7f800000));
-    float x[3];
-    float eps = 0.0f;
-    for (int k = 0; k < 3; ++k) {
-        const double X = N[k] / D;
-        const double E = (kG64 * pN[k] + fabs(X) * e64) / (D - e64) * (1.0 + 1e-12) + fabs(X) * 1e-12;
-        x[k] = (float)X;
-        eps = fmaxf(eps, (float)E + fabsf(x[k]) * kSlack);
+    float4 r = make_float4(0.f, 0.f, 0.f, __int_as_float(0x7f800000));
+    if (D > 2.0 * e64 && pD > kTiny64) {
+        float x[3];
+        float eps = 0.0f, ym = 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            const double X = N[k] / D;
+            const double E = (kG64 * pN[k] + fabs(X) * e64) / (D - e64) * (1.0 + 1e-12) + fabs(X) * 1e-12;
+            x[k] = (float)X;
+            eps = fmaxf(eps, (float)E + fabsf(x[k]) * kSlack);
+            ym = fmaxf(ym, fabsf(x[k]));
+        }
+        r = make_float4(x[0], x[1], x[2], eps + kTiny32);
+        if (eps <= ym * 0x1p-12f) return r;
     }
-    return make_float4(x[0], x[1], x[2], eps + kTiny32);
+    float4 e;
+    if (exact_vertex<kB>(plx, pts, hdr, t, sx, sy, sz, sw, e) && e.w < r.w) r = e;
+    return r;
 }
 
 // ------------------------------------------------------------------ DPP
@@ -440,7 +474,11 @@ struct Group {
     // |N - N~| <= g pN and |D - D~| <= g pD (g = 2e-6), the position error is
     // |y - N~/D~| <= (g pN + |y~| g pD) / (D~ - g pD), plus division rounding.
     // Returns false when the FP32 orientation is not safely positive: the
-    // slot then holds radius -1 and fix_vertices() replaces it.
+    // slot then holds radius -1 and fix_vertices() replaces it.  The tiers
+    // with an FP64 path also leave to it the ill-conditioned vertices, whose
+    // FP32 radius exceeds 2^-12 of the position (nearly parallel or coaxial
+    // planes): one such vertex in a cell inflates its bounds until no node or
+    // candidate can be pruned and every test needs FP64.
     __device__ __forceinline__ bool make_vertex(int slot, uint32_t i, uint32_t j, uint32_t k) {
         const float4 A = S->pl[i], B = S->pl[j], C = S->pl[k];
         const float pHA = S->plx[i].x, pHB = S->plx[j].x, pHC = S->plx[k].x;
@@ -464,7 +502,7 @@ struct Group {
         const uint32_t tri = tri_pack<kB>(i, j, k);
         const float eD = kG32 * pD;
         float4 pos = make_float4(0.f, 0.f, 0.f, -1.0f);
-        const bool ok = D > 2.0f * eD && pD > kTiny32;
+        bool ok = D > 2.0f * eD && pD > kTiny32;
         if (ok) {
             // hardware reciprocals (1 ulp): their error, and the products',
             // lie far inside the kSlack = 1e-6 relative terms of the bound
@@ -475,6 +513,12 @@ struct Group {
             const float ey = (kG32 * pNy + fabsf(y) * eD) * den + fabsf(y) * kSlack;
             const float ez = (kG32 * pNz + fabsf(z) * eD) * den + fabsf(z) * kSlack;
             pos = make_float4(x, y, z, fmaxf(ex, fmaxf(ey, ez)) + kTiny32);
+            if constexpr (kExact) {
+                if (pos.w > 0x1p-12f * fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z)))) {
+                    pos.w = -1.0f;
+                    ok = false;
+                }
+            }
         } else if (D < -eD && pD > kTiny32) {
             count(kStatInconsistent);
         }

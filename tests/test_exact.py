@@ -9,6 +9,7 @@ rationals and the CPU oracle's own expansion arithmetic.
 """
 import ctypes
 import os
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -122,6 +123,101 @@ def test_host_exact_wide_exponent_range():
         tri, q = [1, 2, 3], 4 + int(rng.integers(0, 2))
         got = host_side(lib, pts, w, 0, tri, q)
         assert got == exact_side_fractions(pts, w, 0, tri, q)
+
+
+def host_vertex(lib, seed4, planes, codes, lo, hi):
+    """pg_exact_vertex_host: (y0, y1, y2, w) of the vertex of three planes."""
+    out = np.zeros(4, np.float32)
+    arrs = [np.ascontiguousarray(a, dt) for a, dt in ((seed4, np.float32), (planes, np.float32),
+                                                       (codes, np.int32), (lo, np.float32), (hi, np.float32))]
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    lib.call("pg_exact_vertex_host", *(p(a) for a in arrs), p(out))
+    return out
+
+
+def vertex_fractions(seed4, planes, codes, lo, hi):
+    """(D, y) of the vertex of three planes in rational arithmetic (y None when D == 0)."""
+    s = [Fraction(float(v)) for v in seed4]
+    A, H = [], []
+    for pl, code in zip(planes, codes):
+        if code < 0:
+            a = [Fraction(float(pl[k])) - s[k] for k in range(3)]
+            A.append(a)
+            H.append(sum(x * x for x in a) - (Fraction(float(pl[3])) - s[3]))
+        else:
+            axis, side = int(code) // 2, int(code) % 2
+            a = [Fraction(0)] * 3
+            a[axis] = Fraction(1 if side else -1)
+            A.append(a)
+            H.append(2 * (Fraction(float(hi[axis])) - s[axis]) if side else 2 * (s[axis] - Fraction(float(lo[axis]))))
+    cr = lambda u, v: [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
+    c = [cr(A[1], A[2]), cr(A[2], A[0]), cr(A[0], A[1])]
+    D = sum(A[0][k] * c[0][k] for k in range(3))
+    if D == 0:
+        return D, None
+    return D, [sum(H[t] * c[t][k] for t in range(3)) / D for k in range(3)]
+
+
+def test_host_exact_vertex_position_matches_fractions():
+    """The exact vertex position of the FP64-capable tiers: within its stated
+    radius of the rational vertex, that radius within FP32 rounding, and no
+    vertex (-1) for a non-positive orientation.  Cases: nearly parallel
+    planes of three close points far from the seed (the LiDAR ground cell
+    whose FP32 radius was 25 at |y| = 5), nearly coplanar points, box planes,
+    weights, and random triples."""
+    lib = _lib()
+    h = float.fromhex
+    lidar_seed = np.array([h("0x1.a564bp+3"), h("0x1.d280d8p+4"), h("-0x1.cd0d2cp+0"), 0.0], np.float32)
+    lidar_pts = np.array([[h("0x1.ef3bd8p+3"), h("0x1.0d9fc6p+5"), h("-0x1.cca5d4p+0"), 0.0],
+                          [h("0x1.eaf82cp+3"), h("0x1.0e1dc8p+5"), h("-0x1.ccaaf2p+0"), 0.0],
+                          [h("0x1.f367ccp+3"), h("0x1.0d1118p+5"), h("-0x1.cc955ep+0"), 0.0]], np.float32)
+    lo = np.array([-81.0, -81.0, -2.83], np.float32)
+    hi = np.array([81.0, 81.0, 5.19], np.float32)
+    cases = [(lidar_seed, lidar_pts, np.full(3, -1, np.int32))]
+    rng = np.random.default_rng(33)
+    for trial in range(300):
+        seed = np.array([*rng.uniform(-1, 1, 3), 0.0], np.float32)
+        kind = trial % 4
+        if kind == 0:  # three close points in one direction: nearly parallel planes
+            base = seed[:3] + rng.normal(0, 1, 3) * 3
+            pts = base + rng.normal(0, 1, (3, 3)) * 10.0 ** rng.uniform(-4, -1)
+        elif kind == 1:  # nearly coplanar with the seed: nearly coaxial planes
+            u, v = rng.normal(0, 1, 3), rng.normal(0, 1, 3)
+            coef = rng.normal(0, 1, (3, 2))
+            pts = seed[:3] + coef[:, :1] * u + coef[:, 1:] * v + rng.normal(0, 1, (3, 3)) * 1e-5
+        else:
+            pts = seed[:3] + rng.normal(0, 1, (3, 3))
+        planes = np.zeros((3, 4), np.float32)
+        planes[:, :3] = pts
+        if kind == 3:
+            planes[:, 3] = rng.uniform(0, 0.1, 3)
+            seed[3] = np.float32(0.05)
+        codes = np.full(3, -1, np.int32)
+        if kind == 2 and trial % 8 == 2:
+            codes[rng.integers(0, 3)] = rng.integers(0, 6)
+        cases.append((seed, planes, codes))
+    box = (np.array([-40, -40, -40], np.float32), np.array([40, 40, 40], np.float32))
+    seen_pos = seen_neg = 0
+    for k, (seed, planes, codes) in enumerate(cases):
+        blo, bhi = (lo, hi) if k == 0 else box
+        for order in ((0, 1, 2), (0, 2, 1)):
+            pl, cd = planes[list(order)], codes[list(order)]
+            out = host_vertex(lib, seed, pl, cd, blo, bhi)
+            D, y = vertex_fractions(seed, pl, cd, blo, bhi)
+            if D <= 0:
+                assert out[3] == -1.0, (k, order)
+                seen_neg += 1
+                continue
+            seen_pos += 1
+            w = Fraction(float(out[3]))
+            ym = max(abs(v) for v in y)
+            for j in range(3):
+                assert abs(Fraction(float(out[j])) - y[j]) <= w, (k, order, j)
+            # FP32 rounding of the position plus the 1e-13 term, nothing more
+            assert w <= ym * Fraction(1, 2**23) + Fraction(1, 10**20), (k, order, float(w), float(ym))
+    assert seen_pos >= 290 and seen_neg >= 290
+    out = host_vertex(lib, lidar_seed, lidar_pts, np.full(3, -1, np.int32), lo, hi)
+    assert out[3] < 1e-6 and abs(out[2] - (-1.9751966)) < 1e-6
 
 
 def test_planted_sign_mistake_is_caught():
