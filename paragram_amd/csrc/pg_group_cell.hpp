@@ -478,7 +478,9 @@ struct Group {
     // with an FP64 path also leave to it the ill-conditioned vertices, whose
     // FP32 radius exceeds 2^-12 of the position (nearly parallel or coaxial
     // planes): one such vertex in a cell inflates its bounds until no node or
-    // candidate can be pruned and every test needs FP64.
+    // candidate can be pruned and every test needs FP64.  (The FP32-only
+    // tier keeps them: an FP64 call site there costs it registers on every
+    // cell, and a test such a vertex leaves open sends the cell to the twin.)
     __device__ __forceinline__ bool make_vertex(int slot, uint32_t i, uint32_t j, uint32_t k) {
         const float4 A = S->pl[i], B = S->pl[j], C = S->pl[k];
         const float pHA = S->plx[i].x, pHB = S->plx[j].x, pHC = S->plx[k].x;
@@ -1599,7 +1601,10 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
 __global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3))) cell_group_kernel(const CellParams prm) {
     tier0_body<Tier0, false>(prm);
 }
-__global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) {
+// (3 waves per SIMD, as tier 0: the few registers it then spills cost less
+// than the third wave gains)
+__global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3)))
+cell_group_exact_kernel(const CellParams prm) {
     tier0_body<Tier0x, true>(prm);
 }
 

@@ -99,7 +99,8 @@ __global__ void mlp_features_kernel(int64_t n, const IndexHeader *__restrict__ h
 }
 
 // ------------------------------------------------------------------ inference
-// One wave per 64 points (four 16-point tiles).
+// Each wave loads the weight fragments once, then takes 64-point blocks (four
+// 16-point tiles) in a grid-stride loop.
 __global__ void __launch_bounds__(64) mlp_forward_kernel(int64_t n, const float *__restrict__ X,
                                                          const float *__restrict__ P, float *__restrict__ Y) {
     const int l = threadIdx.x;
@@ -122,8 +123,10 @@ __global__ void __launch_bounds__(64) mlp_forward_kernel(int64_t n, const float 
         a3[k] = pack4(ok ? P[kOffW3 + (in + 0) * kOut + col] : 0.f, ok ? P[kOffW3 + (in + 1) * kOut + col] : 0.f,
                       ok ? P[kOffW3 + (in + 2) * kOut + col] : 0.f, ok ? P[kOffW3 + (in + 3) * kOut + col] : 0.f);
     }
+    const int64_t blocks = (n + 63) / 64;
+    for (int64_t blk = blockIdx.x; blk < blocks; blk += gridDim.x)
     for (int t = 0; t < 4; ++t) {
-        const int64_t pt = (int64_t)blockIdx.x * 64 + 16 * t + col;
+        const int64_t pt = blk * 64 + 16 * t + col;
         float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
         if (pt < n) xv = *(const float4 *)(X + pt * kIn + 4 * kq);
         const bf16x4 bx = pack4(xv.x, xv.y, xv.z, xv.w);
@@ -277,7 +280,9 @@ hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, con
 
 hipError_t mlp_forward(int64_t n, const float *X, const float *params, float *Y, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, n, X, params, Y);
+    const int64_t blocks = (n + 63) / 64;
+    const int64_t grid = blocks < 8192 ? blocks : 8192;  // 32 waves per compute unit, each over many blocks
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)grid), dim3(64), 0, stream, n, X, params, Y);
     return hipGetLastError();
 }
 

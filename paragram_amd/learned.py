@@ -237,7 +237,7 @@ def voronoi_diagram(points: torch.Tensor, *, model: RadiusModel | None = None, l
     return _api.Diagram(adjacency=a, offsets=o, status=s)
 
 
-def training_clouds(count: int = 50, n: int = 1_000_000, seed: int = 100):
+def training_clouds(count: int = 50, n: int = 1_000_000, seed: int = 1000):
     """Seeded clouds drawn like the learned-power configuration (50% uniform +
     50% from 256 Gaussians, weights U[0, 5e-6]), one seed each."""
     for k in range(count):
@@ -260,7 +260,8 @@ def main() -> int:
     t0 = time.perf_counter()
     model = train(lib, list(training_clouds(args.clouds, args.n)), dev, epochs=args.epochs,
                   log=lambda m: print(m, flush=True))
-    model.info.update({"clouds": args.clouds, "points_per_cloud": args.n, "seconds": time.perf_counter() - t0})
+    model.info.update({"clouds": args.clouds, "points_per_cloud": args.n, "seeds": [1000, 1000 + args.clouds - 1],
+                       "seconds": time.perf_counter() - t0})
     save(model, args.out)
     print(f"wrote {args.out} ({model.info})", flush=True)
     return 0
