@@ -116,6 +116,11 @@ def summarize_pmc(fetch_dir: str, write_dir: str, copy_mib: int, workload_key: s
 STAT_KEYS = ("fp64_rechecks", "exact", "unresolved", "promoted", "truncated", "pool_short", "edges")
 
 
+def _progress(msg: str) -> None:
+    """One line per finished phase on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def _time_calls(fn, warmup: int, steps: int):
     """Median and all timings (ms) of steps calls after warmup calls, each
     bracketed by device synchronisations."""
@@ -184,6 +189,7 @@ def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
             exact = {"checked": int(sample.shape[0]), "exact": ok, "frac": ok / sample.shape[0]}
         del out, pts, w
         torch.cuda.empty_cache()
+        _progress(f"extra {key}{' learned' if model is not None else ''}: {ms:.2f} ms per call")
         return desc.format(n=n, seed=seed, total=pts_np.shape[0]), pts_np.shape[0], ms, all_ms, instr, exact
 
     res["c1"] = entry(*single("c1", (10_000, 0), steps=5))
@@ -220,7 +226,50 @@ def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
         exact = {"checked": checked, "exact": ok, "frac": ok / checked}
     res["c4"] = entry(desc.format(n=n_clouds, seed=seed, total=int(clouds[-1])), int(clouds[-1]), ms, all_ms, instr,
                       exact)
+    _progress(f"extra c4: {ms:.2f} ms per batch")
     return res
+
+
+def radius_bound(n=None) -> dict:
+    """What a starting radius can gain on the 10M power configuration: the same diagram built
+    from the kNN pre-clip (plain), from a perfect cube (every cell's exact final half-width, read
+    back from a first build) with and without the pre-clip, from the committed network, and from
+    the box alone; ms per call (median of 3 after one warm-up) and cell-kernel ms.  Every variant
+    is checked byte-identical to the plain output."""
+    import torch
+
+    from paragram_amd import _backend, _engine, learned, workloads
+
+    lib = _backend._C.lib
+    gen, n_def, seed, desc = WORKLOADS["c5"]
+    n = n_def if n is None else n
+    pts_np, w_np = workloads.learned_power(n, seed)
+    dev = torch.device("cuda:0")
+    P = torch.from_numpy(pts_np).to(dev)
+    W = torch.from_numpy(w_np).to(dev).reshape(-1, 1)
+    ext = torch.zeros(n, dtype=torch.float32, device=dev)
+    ref = _engine.build_diagram(lib, P, W, extent=ext)
+    perfect = (ext * 1.001 + 1e-7).contiguous()
+    out = {"workload": desc.format(n=n, seed=seed, total=n)}
+    for label, kw in (("plain (kNN-16 pre-clip)", {}), ("perfect cube + kNN pre-clip", {"init_h": perfect}),
+                      ("learned network + kNN pre-clip", {"radius_model": learned.load()}),
+                      ("box start, no pre-clip", {"knn": 0}), ("perfect cube, no pre-clip", {"knn": 0, "init_h": perfect})):
+        ins = _engine.Instrument()
+        knn = kw.pop("knn", 8)
+
+        def call():
+            ins.cell_events.clear()
+            return _engine.build_diagram(lib, P, W, knn, instrument=ins, **kw)
+
+        ms, all_ms, res = _time_calls(call, 1, 3)
+        same = all(bool(torch.equal(x, y)) for x, y in zip(res, ref))
+        if not same:
+            raise RuntimeError(f"radius bound: {label} differs from the plain output")
+        cell_ms = sum(a.elapsed_time(b) for a, b in ins.cell_events)
+        out[label] = {"ms_per_call": round(ms, 2), "cell_ms": round(cell_ms, 2), "clips": ins.stats["clips"],
+                      "candidates": ins.stats["candidates"], "promoted": ins.stats["promoted"]}
+        _progress(f"radius bound, {label}: {ms:.2f} ms")
+    return out
 
 
 def main() -> int:
@@ -239,6 +288,9 @@ def main() -> int:
                     help="learned: start each cell from the committed radius network's prediction")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed steps, one device copy of this many MiB (PMC calibration)")
+    ap.add_argument("--radius-bound", action="store_true",
+                    help="time the 10M power configuration from a perfect starting radius (each cell's exact "
+                         "final extent) against the plain, learned and box starts, print one JSON line and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this command "
                          "with --copy-mib into profiles/cell_kernel_traffic.json and exit")
@@ -256,6 +308,9 @@ def main() -> int:
                f"--no-cpu --copy-mib {args.copy_mib}")
         rec = summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_mib, wkey, cmd)
         print(json.dumps(rec))
+        return 0
+    if args.radius_bound:
+        print(json.dumps({"radius_bound": radius_bound(args.n if args.workload == "c5" else None)}), flush=True)
         return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
@@ -309,6 +364,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n_cells * args.steps / elapsed / 1e6
+    _progress(f"{args.steps} timed steps: {ms_per_step:.3f} ms per step")
     per_cloud_ms = None
     if clouds is not None:  # the same clouds as one call each, for comparison
         torch.cuda.synchronize()
@@ -317,6 +373,7 @@ def main() -> int:
             _engine.build_diagram(lib, pts[clouds[c]:clouds[c + 1]], None, 8, False, None, None, -1)
         torch.cuda.synchronize()
         per_cloud_ms = (time.perf_counter() - tp) * 1e3
+        _progress(f"the clouds one call each: {per_cloud_ms:.1f} ms")
 
     if args.copy_mib > 0:  # a plain copy of known size for the PMC byte counters
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device=dev)
@@ -350,6 +407,7 @@ def main() -> int:
         th = time.perf_counter()
         h_adj, h_off, h_st, h_stats = hl.diagram(pts_np, w_np, nthreads=cores)
         t_host = time.perf_counter() - th
+        _progress(f"host-core build: {t_host:.2f} s on {cores} cores")
         same = None
         if clouds is None:
             same = bool(np.array_equal(h_adj, adj) and np.array_equal(h_off, off) and np.array_equal(h_st, status))
