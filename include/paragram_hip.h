@@ -313,7 +313,7 @@ int pg_exact_side_host(const float *seed, const float *planes, const int32_t *bo
  * (x, y, z, w), box_codes[3]; out[0..2] = the vertex y = 2 (x - seed) of the
  * three planes rounded to FP32, out[3] = a bound on the rounding error (max
  * norm), or -1 when the orientation D is not positive (no vertex).  The
- * reference computes vertices in FP32 only (voronoi_convex_cell.cuh:226-254);
+ * reference computes vertices in FP32 only (voronoi_convex_cell.cuh:232-254);
  * this is for checking the device code against rational arithmetic. */
 int pg_exact_vertex_host(const float *seed, const float *planes, const int32_t *box_codes, const float *box_lo,
                          const float *box_hi, float *out);

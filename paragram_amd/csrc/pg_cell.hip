@@ -221,8 +221,12 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
     if (hipError_t e = fill_u32(prm.exact_list, count, 0u, stream)) return e;
     hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream, prm);
-    hipLaunchKernelGGL(cell_group_exact_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream,
-                       prm);
+    if (prm.clouds != nullptr)
+        hipLaunchKernelGGL(cell_group_exact_batch_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads),
+                           0, stream, prm);
+    else
+        hipLaunchKernelGGL(cell_group_exact_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0,
+                           stream, prm);
     hipLaunchKernelGGL(cell_group_big_kernel, dim3((unsigned)(a.waves1 * kWave / kT1Threads)), dim3(kT1Threads), 0,
                        stream, prm);
     hipLaunchKernelGGL(cell_group_huge_kernel, dim3((unsigned)kTier2Waves), dim3(kT2Threads), 0, stream, prm);

@@ -306,17 +306,30 @@ template <int C>
 __device__ __forceinline__ float dppf(float x) {
     return __int_as_float(dpp<C>(__float_as_int(x)));
 }
+// Float minima and maxima over a row through an order-preserving integer
+// key (sign-magnitude bits turned into two's complement): each step is one
+// integer min or max with the DPP operand folded in, where fminf/fmaxf of a
+// shuffled value would add two NaN-quieting instructions per step.  The
+// values reduced here are never NaN (+inf is, and orders above every finite
+// value; -0 orders just below +0).
+__device__ __forceinline__ int f2ord(float x) {
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float ord2f(int o) { return __int_as_float(o ^ ((o >> 31) & 0x7fffffff)); }
 __device__ __forceinline__ float row_min(float x) {
-    x = fminf(x, dppf<kDppXor1>(x));
-    x = fminf(x, dppf<kDppXor2>(x));
-    x = fminf(x, dppf<kDppRor4>(x));
-    return fminf(x, dppf<kDppRor8>(x));
+    int o = f2ord(x);
+    o = min(o, dpp<kDppXor1>(o));
+    o = min(o, dpp<kDppXor2>(o));
+    o = min(o, dpp<kDppRor4>(o));
+    return ord2f(min(o, dpp<kDppRor8>(o)));
 }
 __device__ __forceinline__ float row_max(float x) {
-    x = fmaxf(x, dppf<kDppXor1>(x));
-    x = fmaxf(x, dppf<kDppXor2>(x));
-    x = fmaxf(x, dppf<kDppRor4>(x));
-    return fmaxf(x, dppf<kDppRor8>(x));
+    int o = f2ord(x);
+    o = max(o, dpp<kDppXor1>(o));
+    o = max(o, dpp<kDppXor2>(o));
+    o = max(o, dpp<kDppRor4>(o));
+    return ord2f(max(o, dpp<kDppRor8>(o)));
 }
 __device__ __forceinline__ int row_sum(int x) {
     x += dpp<kDppXor1>(x);
@@ -1601,10 +1614,15 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
 __global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3))) cell_group_kernel(const CellParams prm) {
     tier0_body<Tier0, false>(prm);
 }
-// (3 waves per SIMD, as tier 0: the few registers it then spills cost less
-// than the third wave gains)
+__global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const CellParams prm) {
+    tier0_body<Tier0x, true>(prm);
+}
+// The same at 3 waves per SIMD, as tier 0, for batches of clouds, where many
+// cells come here (a fifth of a LiDAR batch): the third wave gains more than
+// the 76 registers it spills cost.  A single cloud sends few cells here, and
+// every persistent wave writes its spills once, so it keeps the uncapped one.
 __global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3)))
-cell_group_exact_kernel(const CellParams prm) {
+cell_group_exact_batch_kernel(const CellParams prm) {
     tier0_body<Tier0x, true>(prm);
 }
 
