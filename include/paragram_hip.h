@@ -329,6 +329,16 @@ int pg_scan_counts(int64_t n, const int32_t *counts, int32_t *offsets, void *tmp
 int pg_chunk_counts(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
                     int32_t *out, void *stream);
 
+/* The cloud-local form of a batch's CSR (offsets [n+1] and adjacency over all
+ * points, as the batch calls return them): local_adjacency[e] = adjacency[e]
+ * minus the first point of its row's cloud, and local_offsets [n + n_clouds]
+ * = each cloud's offsets (n_c + 1 of them, from 0) one cloud after another.
+ * cloud_offsets: device int64 [n_clouds + 1], the batch's (0 .. n,
+ * non-decreasing).  No counterpart in the reference (one cloud per call);
+ * the per-cloud results of paragram_amd.batched_voronoi_diagram. */
+int pg_batch_localize(int64_t n, int32_t n_clouds, const int64_t *cloud_offsets, const int32_t *offsets,
+                      const int32_t *adjacency, int32_t *local_adjacency, int32_t *local_offsets, void *stream);
+
 /* Copies the rows of sorted cells [begin, end) from rows + row_offsets[i]
  * to packed + local_offsets[i - begin]. */
 int pg_pack_rows(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,

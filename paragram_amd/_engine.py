@@ -59,6 +59,7 @@ class HipLibrary:
             "pg_exact_side": ([i64, vp, i64, vp, vp, vp], ctypes.c_int),
             "pg_exact_side_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_exact_vertex_host": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_batch_localize": ([i64, i32, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_index_layout": ([i64, u64p], ctypes.c_int),
             "pg_batch_set_offsets": ([i64, i32, vp, vp, u64, vp], ctypes.c_int),
             "pg_build_batch_index_resident": ([i64, i32, vp, vp, i32, vp, u64, vp], ctypes.c_int),
@@ -370,40 +371,44 @@ def knn(lib: HipLibrary, points: torch.Tensor, k: int = 8, leaf_size: int = -1):
     return ids, d2
 
 
-def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds, cloud_entries=None):
+def split_batch(adjacency: torch.Tensor, offsets: torch.Tensor, status: torch.Tensor, clouds, cloud_entries=None,
+                lib: HipLibrary | None = None):
     """Per-cloud ``(adjacency, offsets, status)`` of a batch diagram, with
-    cloud-local ids; views into two new device arrays.  ``cloud_entries``
-    (host, optional): each cloud's adjacency entry count, else read once."""
+    cloud-local ids; views into two new device arrays written by one
+    ``pg_batch_localize`` pass.  ``cloud_entries`` (host, optional): each
+    cloud's adjacency entry count, else read once."""
+    if lib is None:
+        from ._backend import _C
+
+        if _C is None:
+            raise RuntimeError("libparagram_hip.so is not loaded")
+        lib = _C.lib
     dev = offsets.device
     clouds = np.ascontiguousarray(clouds, dtype=np.int64)
     b = clouds.shape[0] - 1
-    starts = torch.from_numpy(clouds[:-1].copy()).to(dev)
-    ends = torch.from_numpy(clouds[1:].copy()).to(dev)
-    off64 = offsets.to(torch.int64)
-    seg = off64[ends] - off64[starts]  # entries per cloud
+    n = int(offsets.numel()) - 1
     e_total = int(adjacency.numel())
-    local_adj = (adjacency.to(torch.int64) - torch.repeat_interleave(starts, seg, output_size=e_total)).to(torch.int32)
-    # cloud c's offsets are offsets[a_c .. b_c] - offsets[a_c], laid out one
-    # cloud after another (n_c + 1 entries each)
-    lens = clouds[1:] - clouds[:-1] + 1
-    firsts = np.concatenate([[0], np.cumsum(lens)[:-1]])
-    pos = torch.arange(int(lens.sum()), device=dev)
-    cid = torch.searchsorted(torch.from_numpy(firsts).to(dev), pos, right=True) - 1
-    row = pos - torch.from_numpy(firsts).to(dev)[cid] + starts[cid]
-    local_off = (off64[row] - off64[starts[cid]]).to(torch.int32)
-    out = []
+    i32 = dict(dtype=torch.int32, device=dev)
+    cl_dev = torch.from_numpy(clouds).to(dev)
+    local_adj = torch.empty(max(e_total, 1), **i32)
+    local_off = torch.empty(n + b, **i32)
+    adj_in = adjacency if e_total > 0 else torch.zeros(1, **i32)
+    with torch.cuda.device(dev):
+        sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        lib.call("pg_batch_localize", n, b, _p(cl_dev), _p(offsets), _p(adj_in), _p(local_adj), _p(local_off), sp)
+    # cloud c's offsets: local_off[firsts[c] .. firsts[c] + n_c] (n_c + 1 entries)
+    firsts = clouds[:-1] + np.arange(b, dtype=np.int64)
+    if cloud_entries is None:  # adjacency slices need the per-cloud entry counts on the host (one read)
+        bounds = offsets[cl_dev].to(torch.int64).cpu().numpy()
+        seg_host = bounds[1:] - bounds[:-1]
+    else:
+        seg_host = np.asarray(cloud_entries)
+    res = []
     e0 = 0
-    seg_host = None
     for c in range(b):
         a, z = int(clouds[c]), int(clouds[c + 1])
-        o = local_off[int(firsts[c]):int(firsts[c]) + (z - a) + 1]
-        out.append((o, status[a:z]))
-    # adjacency slices need the per-cloud entry counts on the host (one read)
-    seg_host = seg.cpu().numpy() if cloud_entries is None else np.asarray(cloud_entries)
-    res = []
-    for c in range(b):
         m = int(seg_host[c])
-        res.append((local_adj[e0:e0 + m], out[c][0], out[c][1]))
+        res.append((local_adj[e0:e0 + m], local_off[int(firsts[c]):int(firsts[c]) + (z - a) + 1], status[a:z]))
         e0 += m
     return res
 

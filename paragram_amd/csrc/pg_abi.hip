@@ -505,6 +505,17 @@ int pg_chunk_counts(int64_t n, const void *workspace, int64_t begin, int64_t end
                      "chunk_counts");
 }
 
+int pg_batch_localize(int64_t n, int32_t n_clouds, const int64_t *cloud_offsets, const int32_t *offsets,
+                      const int32_t *adjacency, int32_t *local_adjacency, int32_t *local_offsets, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (!cloud_offsets || !offsets || !local_offsets || (n > 0 && (!adjacency || !local_adjacency)))
+        return fail(1, "invalid argument");
+    return check_hip(pg::batch_localize(n, n_clouds, cloud_offsets, offsets, adjacency, local_adjacency,
+                                        local_offsets, as_stream(stream)),
+                     "batch_localize");
+}
+
 int pg_pack_rows(int64_t n, const void *workspace, int64_t begin, int64_t end, const int32_t *counts,
                  const int32_t *local_offsets, const int32_t *rows, const int64_t *row_offsets, int32_t *packed,
                  void *stream) {

@@ -249,7 +249,54 @@ __global__ void union_phase3_kernel(int64_t n, const int32_t *__restrict__ new_o
     }
 }
 
+// Cloud-local form of a batch's CSR (every neighbour lies in its row's
+// cloud): ids minus the cloud's first point, and each cloud's offsets
+// (n_c + 1 of them, from 0) laid out one cloud after another.  Offsets out
+// of order only give wrong values: every index is clamped.
+__device__ __forceinline__ int cloud_of(const int64_t *off, int n_clouds, int64_t u) {
+    int lo = 0, hi = n_clouds - 1;  // largest c with off[c] <= u
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ void batch_local_adj_kernel(int64_t n, int n_clouds, const int64_t *__restrict__ cloud_off,
+                                       const int32_t *__restrict__ offsets, const int32_t *__restrict__ adj,
+                                       int32_t *__restrict__ out) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int32_t s = (int32_t)cloud_off[cloud_of(cloud_off, n_clouds, u)];
+    for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) out[e] = adj[e] - s;
+}
+__global__ void batch_local_offsets_kernel(int64_t n, int n_clouds, const int64_t *__restrict__ cloud_off,
+                                           const int32_t *__restrict__ offsets, int32_t *__restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n + n_clouds) return;
+    int lo = 0, hi = n_clouds - 1;  // largest c with off[c] + c <= p
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cloud_off[mid] + mid <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    int64_t u = p - lo, a = cloud_off[lo];
+    u = u < 0 ? 0 : (u > n ? n : u);
+    a = a < 0 ? 0 : (a > n ? n : a);
+    out[p] = offsets[u] - offsets[a];
+}
+
 static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+hipError_t batch_localize(int64_t n, int n_clouds, const int64_t *cloud_off, const int32_t *offsets,
+                          const int32_t *adj, int32_t *local_adj, int32_t *local_offsets, hipStream_t stream) {
+    if (n > 0)
+        hipLaunchKernelGGL(batch_local_adj_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, n, n_clouds, cloud_off,
+                           offsets, adj, local_adj);
+    hipLaunchKernelGGL(batch_local_offsets_kernel, dim3(grid_for(n + n_clouds, 256)), dim3(256), 0, stream, n, n_clouds,
+                       cloud_off, offsets, local_offsets);
+    return hipGetLastError();
+}
 
 hipError_t gather_counts_sorted(const int32_t *counts, const uint32_t *perm, int64_t begin, int64_t end,
                                 uint32_t *out, hipStream_t stream) {
