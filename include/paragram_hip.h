@@ -236,7 +236,9 @@ int pg_mlp_train_step(int64_t n, const float *features, const float *targets, co
 int pg_batch_workspace_bytes(int64_t n, int32_t n_clouds, uint64_t *bytes);
 
 /* Builds the batch index (per-cloud AABBs and boxes, one sort and hierarchy
- * over keys cloud id || per-cloud Morton code, each cloud's subtree root). */
+ * over keys cloud id || per-cloud Morton code, each cloud's subtree root).
+ * The cloud offsets (host array) are copied into the workspace before the
+ * call returns. */
 int pg_build_batch_index(int64_t n, int32_t n_clouds, const float *points, const float *weights,
                          const int64_t *cloud_offsets, int32_t leaf_size, void *workspace, uint64_t workspace_bytes,
                          void *stream);
@@ -247,7 +249,10 @@ int pg_batch_quads(int32_t n_clouds, const int64_t *cloud_offsets, int64_t begin
 
 /* pg_compute_cells over the cells [begin, end) of a batch (any range of the
  * batch, so the chunk plan applies unchanged); the persistent waves take one
- * quad per round from a global counter.  guesses / guess_offsets: optional,
+ * quad per round from a global counter.  cloud_offsets: the host array given
+ * to pg_build_batch_index for this workspace (checked here; the work quads
+ * are computed on the device from the workspace's copy, so nothing is copied
+ * from the host).  guesses / guess_offsets: optional,
  * as for pg_compute_cells (original ids of the same cloud), or rows of
  * pg_knn_guesses (guess_stride = k, guess_d2 = its kth_d2 or null). */
 int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, const int64_t *cloud_offsets,

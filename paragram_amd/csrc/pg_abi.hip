@@ -142,8 +142,10 @@ int pg_build_batch_index(int64_t n, int32_t n_clouds, const float *points, const
     if (leaf < 1) leaf = 1;
     if (leaf > pg::kMaxLeaf) leaf = pg::kMaxLeaf;
     int64_t *dev_off = (int64_t *)((char *)workspace + B.offsets);
+    // the host array may go when this returns: the copy completes first
     hipError_t e = hipMemcpyAsync(dev_off, cloud_offsets, 8 * ((size_t)n_clouds + 1), hipMemcpyHostToDevice,
                                   as_stream(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
     if (e != hipSuccess) return check_hip(e, "build_batch_index");
     return check_hip(pg::build_batch_index(n, n_clouds, points, weights, dev_off, leaf, workspace, as_stream(stream)),
                      "build_batch_index");
@@ -376,13 +378,12 @@ int pg_compute_cells_batch(int64_t n, const void *workspace, int32_t n_clouds, c
     if (int r = check_batch_cells(n, begin, end, workspace, rows, row_offsets, counts, status, stats, scratch,
                                   scratch_bytes, guesses, guess_offsets, guess_stride, waves))
         return r;
+    // the work quads from the cloud offsets that pg_build_batch_index left in
+    // the workspace (checked above against the host copy), on the stream
     const pg::BatchLayout B = pg::make_batch_layout(n, n_clouds);
-    std::vector<int64_t> quads((size_t)n_clouds + 1);
-    if (int r = pg_batch_quads(n_clouds, cloud_offsets, begin, end, quads.data())) return r;
-    // the copy is ordered before the kernels on the stream; hipMemcpyAsync
-    // returns once the pageable source has been staged
-    hipError_t e = hipMemcpyAsync((char *)workspace + B.quads, quads.data(), 8 * quads.size(), hipMemcpyHostToDevice,
-                                  as_stream(stream));
+    char *ws = (char *)const_cast<void *>(workspace);
+    hipError_t e = pg::batch_quads((const int64_t *)(ws + B.offsets), n_clouds, begin, end, (int64_t *)(ws + B.quads),
+                                   as_stream(stream));
     if (e != hipSuccess) return check_hip(e, "compute_cells_batch");
     return cells_batch(n, workspace, n_clouds, begin, end, guesses, guess_offsets, guess_stride, guess_d2, rows,
                        rows_elems, row_offsets, counts, status, stats, scratch, scratch_bytes, waves, cell_cycles,

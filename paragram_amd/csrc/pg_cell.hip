@@ -212,7 +212,9 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;
     const int64_t blocks = max_blocks < blocks_needed ? max_blocks : blocks_needed;
     // A tier-0 wave takes the pool in chunks, one device-wide atomic per
-    // several rounds; what the waves leave unused is at most 1/16 of the pool.
+    // several rounds; what the waves of tier 0 leave unused is at most 1/16
+    // of the pool, and as much again for its FP64 twin (same grid, same
+    // chunks).  A pool that runs short is reported and grown, never silent.
     {
         const uint64_t waves0 = (uint64_t)(blocks > 0 ? blocks : 1) * (kT0Threads / kWave);
         const uint64_t per_wave = prm.rows_elems / (16 * waves0);

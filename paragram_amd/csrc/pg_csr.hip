@@ -286,7 +286,47 @@ __global__ void batch_local_offsets_kernel(int64_t n, int n_clouds, const int64_
     out[p] = offsets[u] - offsets[a];
 }
 
+// Work quads of the cells [begin, end) of a batch from its cloud offsets on
+// the device (pg_batch_quads's table, without a host copy): quads[c] = the
+// first quad of cloud c, quads[n_clouds] = all quads.  One block: each
+// thread sums a run of clouds, then the runs are scanned in LDS.
+constexpr int kQuadThreads = 1024;
+__global__ void __launch_bounds__(kQuadThreads) batch_quads_kernel(const int64_t *__restrict__ off, int n_clouds,
+                                                                   int64_t begin, int64_t end,
+                                                                   int64_t *__restrict__ quads) {
+    __shared__ int64_t part[kQuadThreads];
+    const int per = (n_clouds + kQuadThreads - 1) / kQuadThreads;
+    const int c0 = threadIdx.x * per, c1 = min(c0 + per, n_clouds);
+    auto quads_of = [&](int c) {
+        const int64_t lo = off[c] > begin ? off[c] : begin;
+        const int64_t hi = off[c + 1] < end ? off[c + 1] : end;
+        return hi > lo ? (hi - lo + 3) / 4 : (int64_t)0;
+    };
+    int64_t sum = 0;
+    for (int c = c0; c < c1; ++c) sum += quads_of(c);
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < kQuadThreads; d <<= 1) {  // inclusive scan of the runs
+        const int64_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int64_t q = part[threadIdx.x] - sum;
+    for (int c = c0; c < c1; ++c) {
+        quads[c] = q;
+        q += quads_of(c);
+    }
+    if (threadIdx.x == kQuadThreads - 1) quads[n_clouds] = part[threadIdx.x];
+}
+
 static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+hipError_t batch_quads(const int64_t *off, int n_clouds, int64_t begin, int64_t end, int64_t *quads,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(batch_quads_kernel, dim3(1), dim3(kQuadThreads), 0, stream, off, n_clouds, begin, end, quads);
+    return hipGetLastError();
+}
 
 hipError_t batch_localize(int64_t n, int n_clouds, const int64_t *cloud_off, const int32_t *offsets,
                           const int32_t *adj, int32_t *local_adj, int32_t *local_offsets, hipStream_t stream) {

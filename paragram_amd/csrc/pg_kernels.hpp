@@ -67,6 +67,8 @@ hipError_t emit_rows(int64_t n, const uint32_t *inv, const int32_t *counts, cons
 hipError_t check_symmetry(int64_t n, const uint32_t *perm, const uint32_t *inv, const int32_t *counts, const int32_t *src,
                           const int64_t *src_offsets, int64_t src_stride, const int32_t *offsets,
                           unsigned long long *stats, hipStream_t stream);
+hipError_t batch_quads(const int64_t *off, int n_clouds, int64_t begin, int64_t end, int64_t *quads,
+                       hipStream_t stream);
 hipError_t batch_localize(int64_t n, int n_clouds, const int64_t *cloud_off, const int32_t *offsets,
                           const int32_t *adj, int32_t *local_adj, int32_t *local_offsets, hipStream_t stream);
 hipError_t pack_rows(int64_t begin, int64_t end, const uint32_t *perm, const int32_t *counts,
