@@ -216,6 +216,17 @@ def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
         return _engine.split_batch(*out, clouds)
 
     ms, all_ms, parts = _time_calls(batch_call, 1, 3)
+    # the north star's chosen form: the same batch started from the learned radius
+    instr_l = _engine.Instrument()
+
+    def batch_learned():
+        instr_l.cell_events.clear()
+        out = _engine.build_diagram(lib, pts, None, 8, False, None, None, -1, instrument=instr_l, clouds=clouds,
+                                    radius_model=model)
+        return _engine.split_batch(*out, clouds)
+
+    ms_l, all_ms_l, _ = _time_calls(batch_learned, 1, 3)
+    _progress(f"extra c4 learned: {ms_l:.2f} ms per batch")
     exact = None
     if cpu_check:
         ok = checked = 0
@@ -227,6 +238,10 @@ def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
         exact = {"checked": checked, "exact": ok, "frac": ok / checked}
     res["c4"] = entry(desc.format(n=n_clouds, seed=seed, total=int(clouds[-1])), int(clouds[-1]), ms, all_ms, instr,
                       exact)
+    res["c4_learned"] = entry(desc.format(n=n_clouds, seed=seed, total=int(clouds[-1]))
+                              + " (learned-radius form: committed radius_mlp.npz)", int(clouds[-1]), ms_l, all_ms_l,
+                              instr_l, None)
+    res["c4_learned"]["learned_over_plain"] = round(ms / ms_l, 4)
     _progress(f"extra c4: {ms:.2f} ms per batch")
     return res
 

@@ -203,6 +203,12 @@ int pg_mlp_param_count(void);
 int pg_mlp_features(int64_t n, const void *workspace, const int32_t *ids, const float *d2, float *features,
                     float *log_d8, void *stream);
 
+/* pg_mlp_features over a batch index (pg_build_batch_index) with
+ * pg_knn_batch's ids and d2: the hull features use each point's own cloud's
+ * AABB. */
+int pg_mlp_features_batch(int64_t n, const void *workspace, int32_t n_clouds, const int32_t *ids, const float *d2,
+                          float *features, float *log_d8, void *stream);
+
 /* out[i*2 + 0..1] = network(features[i*16 .. i*16 + 15]). */
 int pg_mlp_forward(int64_t n, const float *features, const float *params, float *out, void *stream);
 

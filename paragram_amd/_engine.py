@@ -81,6 +81,7 @@ class HipLibrary:
             "pg_knn_guesses": ([i64, vp, i32, i32, vp, vp, vp], ctypes.c_int),
             "pg_mlp_param_count": ([], ctypes.c_int),
             "pg_mlp_features": ([i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "pg_mlp_features_batch": ([i64, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
             "pg_mlp_forward": ([i64, vp, vp, vp, vp], ctypes.c_int),
             "pg_mlp_train_scratch_elems": ([i64, u64p], ctypes.c_int),
             "pg_mlp_train_step": ([i64, vp, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp, ctypes.c_float, i32, vp],
@@ -173,10 +174,10 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
             lib.call("pg_build_batch_index", n, n_clouds, _p(points), _p(weights), c_off, int(leaf_size), _p(ws),
                      ws_bytes, sp)
-        if radius_model is not None and clouds is None:
+        if radius_model is not None:
             from . import learned
 
-            init_h = learned.predict_on_index(lib, radius_model, ws, n, dev, sp)
+            init_h = learned.predict_on_index(lib, radius_model, ws, n, dev, sp, 0 if clouds is None else n_clouds)
         if debug:
             torch.cuda.synchronize(dev)
             print("BVH Creation time: %f ms" % ((time.perf_counter() - t0) * 1e3))

@@ -185,6 +185,17 @@ int pg_mlp_features(int64_t n, const void *workspace, const int32_t *ids, const 
     return check_hip(pg::mlp_features(n, workspace, ids, d2, features, log_d8, as_stream(stream)), "mlp_features");
 }
 
+int pg_mlp_features_batch(int64_t n, const void *workspace, int32_t n_clouds, const int32_t *ids, const float *d2,
+                          float *features, float *log_d8, void *stream) {
+    if (int r = check_n(n)) return r;
+    if (n_clouds < 1 || n_clouds > pg::kMaxClouds) return fail(1, "number of clouds must be in [1, 4096]");
+    if (n == 0) return 0;
+    if (!workspace || !ids || !d2 || !features || !log_d8)
+        return fail(1, "workspace, ids, d2, features and log_d8 must be non-null");
+    return check_hip(pg::mlp_features(n, workspace, ids, d2, features, log_d8, as_stream(stream), n_clouds),
+                     "mlp_features_batch");
+}
+
 int pg_mlp_forward(int64_t n, const float *features, const float *params, float *out, void *stream) {
     if (n < 0) return fail(1, "number of rows must be non-negative");
     if (n == 0) return 0;

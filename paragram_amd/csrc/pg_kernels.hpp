@@ -19,7 +19,7 @@ hipError_t leaf_stats(int64_t n, const void *ws, unsigned long long *out, hipStr
 hipError_t knn(int64_t n, void *index_ws, int k, int32_t *ids, float *d2, hipStream_t stream,
                int n_clouds = 0, bool sorted_rows = false);
 hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
-                        float *logd8, hipStream_t stream);
+                        float *logd8, hipStream_t stream, int n_clouds = 0);
 hipError_t mlp_forward(int64_t n, const float *X, const float *params, float *Y, hipStream_t stream);
 int mlp_train_blocks(int64_t n);
 hipError_t mlp_train_step(int64_t n, const float *X, const float *T, const float *mask, float count, float *params,

@@ -56,13 +56,26 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
 //   10   (w_s - mean w_nn) / d_8^2    11  (max w_nn - w_s) / d_8^2
 //   12   (w_s - min w_nn) / d_8^2     13  std w_nn / d_8^2
 //   14   (d_8 - d_1) / d_8            15  1
-// and log(d_8), the scale the prediction is relative to.
+// and log(d_8), the scale the prediction is relative to.  In a batch
+// (clouds non-null) the AABB is that of the point's own cloud.
 __global__ void mlp_features_kernel(int64_t n, const IndexHeader *__restrict__ hdr, const float4 *__restrict__ pts,
                                     const uint32_t *__restrict__ inv, const int32_t *__restrict__ ids,
-                                    const float *__restrict__ d2, float *__restrict__ X, float *__restrict__ logd8) {
+                                    const float *__restrict__ d2, float *__restrict__ X, float *__restrict__ logd8,
+                                    const IndexHeader *__restrict__ clouds, const int64_t *__restrict__ cloud_cells,
+                                    int n_clouds) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 s = pts[inv[i]];
+    const uint32_t si = inv[i];
+    const float4 s = pts[si];
+    if (clouds != nullptr) {
+        int lo = 0, hi = n_clouds - 1;  // largest c with cloud_cells[c] <= si
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cloud_cells[mid] <= (int64_t)si) lo = mid;
+            else hi = mid - 1;
+        }
+        hdr = &clouds[lo];
+    }
     float d[8];
     for (int t = 0; t < 8; ++t) d[t] = sqrtf(fminf(d2[i * 8 + t], 1e30f));
     const float d8 = fmaxf(d[7], 1e-30f);
@@ -268,13 +281,20 @@ __global__ void mlp_adam_kernel(int blocks, const float *__restrict__ part, floa
 }
 
 hipError_t mlp_features(int64_t n, const void *index_ws, const int32_t *ids, const float *d2, float *X,
-                        float *logd8, hipStream_t stream) {
+                        float *logd8, hipStream_t stream, int n_clouds) {
     if (n <= 0) return hipSuccess;
     const IndexLayout L = make_index_layout(n);
     const char *ws = (const char *)index_ws;
+    const IndexHeader *clouds = nullptr;
+    const int64_t *cloud_cells = nullptr;
+    if (n_clouds > 0) {
+        const BatchLayout B = make_batch_layout(n, n_clouds);
+        clouds = (const IndexHeader *)(ws + B.clouds);
+        cloud_cells = (const int64_t *)(ws + B.offsets);
+    }
     hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n,
                        (const IndexHeader *)(ws + L.header), (const float4 *)(ws + L.points),
-                       (const uint32_t *)(ws + L.inv), ids, d2, X, logd8);
+                       (const uint32_t *)(ws + L.inv), ids, d2, X, logd8, clouds, cloud_cells, n_clouds);
     return hipGetLastError();
 }
 

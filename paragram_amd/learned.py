@@ -183,17 +183,22 @@ def train(lib, clouds, dev, epochs: int = 3, batch: int = 8192, lr: float = 2e-3
     return RadiusModel(tr.P.cpu().numpy(), margin, info)
 
 
-def predict_on_index(lib, model: RadiusModel, ws: torch.Tensor, n: int, dev, sp) -> torch.Tensor:
-    """Predicted half-widths from an index already built (pg_build_index): kNN,
-    features and the network on that workspace, no second index."""
+def predict_on_index(lib, model: RadiusModel, ws: torch.Tensor, n: int, dev, sp, n_clouds: int = 0) -> torch.Tensor:
+    """Predicted half-widths from an index already built (pg_build_index, or
+    pg_build_batch_index with n_clouds > 0: neighbours and hull features per
+    cloud): kNN, features and the network on that workspace, no second index."""
     p = _engine._p
     ids = torch.empty((n, 8), dtype=torch.int32, device=dev)
     d2 = torch.empty((n, 8), dtype=torch.float32, device=dev)
     X = torch.empty((n, N_FEATURES), dtype=torch.float32, device=dev)
     logd8 = torch.empty(n, dtype=torch.float32, device=dev)
     Y = torch.empty((n, 2), dtype=torch.float32, device=dev)
-    lib.call("pg_knn", n, p(ws), 8, p(ids), p(d2), sp)
-    lib.call("pg_mlp_features", n, p(ws), p(ids), p(d2), p(X), p(logd8), sp)
+    if n_clouds > 0:
+        lib.call("pg_knn_batch", n, p(ws), n_clouds, 8, p(ids), p(d2), sp)
+        lib.call("pg_mlp_features_batch", n, p(ws), n_clouds, p(ids), p(d2), p(X), p(logd8), sp)
+    else:
+        lib.call("pg_knn", n, p(ws), 8, p(ids), p(d2), sp)
+        lib.call("pg_mlp_features", n, p(ws), p(ids), p(d2), p(X), p(logd8), sp)
     lib.call("pg_mlp_forward", n, p(X), p(model.device_params(dev)), p(Y), sp)
     h = torch.exp(Y[:, 0] + logd8 + model.margin)
     return torch.where(interior(X), h, torch.zeros_like(h)).contiguous()
@@ -235,6 +240,33 @@ def voronoi_diagram(points: torch.Tensor, *, model: RadiusModel | None = None, l
     model = model or load()
     a, o, s = _engine.build_diagram(lib, points, None, 8, False, None, None, leaf_size, radius_model=model)
     return _api.Diagram(adjacency=a, offsets=o, status=s)
+
+
+def batched_voronoi_diagram(points: torch.Tensor, cloud_offsets, *, model: RadiusModel | None = None,
+                            leaf_size: int = -1) -> list:
+    """``paragram_amd.batched_voronoi_diagram`` with each cell started from the
+    learned radius (neighbours and features per cloud; same result)."""
+    points = _api._validate_points(points)
+    off = _api._validate_cloud_offsets(cloud_offsets, points.shape[0])
+    lib = _lib()
+    model = model or load()
+    a, o, s = _engine.build_diagram(lib, points, None, 8, False, None, None, leaf_size, clouds=off,
+                                    radius_model=model)
+    return [_api.Diagram(adjacency=x, offsets=y, status=z) for x, y, z in _engine.split_batch(a, o, s, off, lib=lib)]
+
+
+def batched_power_diagram(points: torch.Tensor, weights: torch.Tensor, cloud_offsets, *,
+                          model: RadiusModel | None = None, leaf_size: int = -1) -> list:
+    """``paragram_amd.batched_power_diagram`` with each cell started from the
+    learned radius (same result)."""
+    points = _api._validate_points(points)
+    weights = _api._validate_weights(weights, points.shape[0], points.device)
+    off = _api._validate_cloud_offsets(cloud_offsets, points.shape[0])
+    lib = _lib()
+    model = model or load()
+    a, o, s = _engine.build_diagram(lib, points, weights, 8, False, None, None, leaf_size, clouds=off,
+                                    radius_model=model)
+    return [_api.Diagram(adjacency=x, offsets=y, status=z) for x, y, z in _engine.split_batch(a, o, s, off, lib=lib)]
 
 
 def training_clouds(count: int = 50, n: int = 1_000_000, seed: int = 1000):

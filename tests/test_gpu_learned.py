@@ -185,3 +185,63 @@ def test_c5_learned_form_with_committed_model(gpu):
     bad = sum(int(st[i] != ref.status[k] or not np.array_equal(adj[off[i]:off[i + 1]], ref.row(k)))
               for k, i in enumerate(sample))
     assert bad == 0, f"{bad} of {sample.shape[0]} sampled cells differ"
+
+
+def test_learned_batched_form_equals_plain_batch(gpu):
+    """The learned radius on batches (neighbours and hull features per cloud):
+    32 LiDAR clouds (seed 3) and a ragged batch of power clouds give exactly
+    the plain batched results, cloud by cloud, and the prediction on the
+    batch index equals each cloud's prediction on its own index."""
+    import ctypes
+
+    import torch
+
+    import paragram_amd
+    from paragram_amd import _engine, learned
+
+    log("learned radius on batches: 32 LiDAR clouds and 6 power clouds vs the plain batched calls")
+    model = learned.load()
+    cl = workloads.lidar_batch(3, n_clouds=256, max_clouds=32)
+    off = np.concatenate([[0], np.cumsum([c.shape[0] for c in cl])]).astype(np.int64)
+    P = torch.from_numpy(np.concatenate(cl)).to(gpu)
+    got = learned.batched_voronoi_diagram(P, off, model=model)
+    want = paragram_amd.batched_voronoi_diagram(P, off)
+    assert len(got) == len(want) == 32
+    for c, (g, w) in enumerate(zip(got, want)):
+        for name in ("adjacency", "offsets", "status"):
+            assert torch.equal(getattr(g, name), getattr(w, name)), (c, name)
+    rng = np.random.default_rng(11)
+    sizes = [3000, 1, 0, 5000, 17, 8000]
+    pts, ws = [], []
+    for s in sizes:
+        pw = workloads.learned_power(max(s, 1), int(rng.integers(1 << 30)))
+        pts.append(pw[0][:s])
+        ws.append(pw[1][:s])
+    off2 = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    P2 = torch.from_numpy(np.concatenate(pts)).to(gpu)
+    W2 = torch.from_numpy(np.concatenate(ws)).to(gpu)
+    got = learned.batched_power_diagram(P2, W2, off2, model=model)
+    want = paragram_amd.batched_power_diagram(P2, W2, off2)
+    for c, (g, w) in enumerate(zip(got, want)):
+        for name in ("adjacency", "offsets", "status"):
+            assert torch.equal(getattr(g, name), getattr(w, name)), (c, name)
+    # the batch prediction itself (per-cloud kNN and hull features on the batch
+    # index) equals each cloud's prediction on its own index
+    lib = learned._lib()
+    sp = ctypes.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    n = P2.shape[0]
+    ws_bytes = lib.u64("pg_batch_workspace_bytes", n, len(sizes))
+    wsb = torch.empty(ws_bytes, dtype=torch.uint8, device=gpu)
+    lib.call("pg_build_batch_index", n, len(sizes), _engine._p(P2), _engine._p(W2), ctypes.c_void_p(off2.ctypes.data),
+             -1, _engine._p(wsb), ws_bytes, sp)
+    hb = learned.predict_on_index(lib, model, wsb, n, gpu, sp, len(sizes)).cpu().numpy()
+    for c, sz in enumerate(sizes):
+        if sz < 17:
+            continue
+        a = int(off2[c])
+        Pc, Wc = P2[a:a + sz].contiguous(), W2[a:a + sz].contiguous()
+        wb = lib.u64("pg_index_workspace_bytes", sz)
+        wsc = torch.empty(wb, dtype=torch.uint8, device=gpu)
+        lib.call("pg_build_index", sz, _engine._p(Pc), _engine._p(Wc), -1, _engine._p(wsc), wb, sp)
+        hc = learned.predict_on_index(lib, model, wsc, sz, gpu, sp).cpu().numpy()
+        assert np.allclose(hb[a:a + sz], hc, rtol=1e-5, atol=0), c
