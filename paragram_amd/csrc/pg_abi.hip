@@ -37,7 +37,7 @@ int check_n(int64_t n) {
 // the process ends.  In host code `scratch` names a host-side stand-in, not
 // the device array: its device address comes from hipGetSymbolAddress, once
 // per device, and only that address is handed to kernels.
-constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+constexpr uint64_t kNoAllocScratchBytes = 24ull << 20;
 __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -582,9 +582,18 @@ int pg_fill_u32(uint32_t *p, int64_t n, uint32_t v, void *stream) {
     return check_hip(pg::fill_u32(p, n, v, as_stream(stream)), "fill");
 }
 
+// The most cells one launch of pg_compute_cells_noalloc covers: the largest
+// range whose cell scratch fits the static array.
+int64_t noalloc_max_cells() {
+    int64_t c = (int64_t)(kNoAllocScratchBytes / 18);
+    while (c > 0 && pg::cell_scratch_bytes(c) > kNoAllocScratchBytes) --c;
+    while (pg::cell_scratch_bytes(c + 1) <= kNoAllocScratchBytes) ++c;
+    return c;
+}
+
 int pg_noalloc_max_cells(int64_t *cells) {
     if (!cells) return fail(1, "cells is null");
-    *cells = (int64_t)((kNoAllocScratchBytes - 256) / 12);
+    *cells = noalloc_max_cells();
     return 0;
 }
 
@@ -598,8 +607,8 @@ int pg_compute_cells_noalloc(int64_t n, const void *workspace, int64_t begin, in
     if (begin == end) return 0;
     const uint64_t need = pg::cell_scratch_bytes(end - begin);
     if (need > kNoAllocScratchBytes)
-        return fail(1, "cell range too large for the static cell scratch (at most 1398080 cells per launch; "
-                       "split the range)");
+        return fail(1, "cell range too large for the static cell scratch (at most " +
+                           std::to_string(noalloc_max_cells()) + " cells per launch; split the range)");
     void *ptr = nullptr;
     if (int r = noalloc_scratch(&ptr)) return r;
     return pg_compute_cells_ex(n, workspace, begin, end, guesses, guess_offsets, guesses_sorted, guess_stride,

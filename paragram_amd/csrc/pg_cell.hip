@@ -69,6 +69,8 @@ struct CellParams {
     uint32_t *huge_list;          // tier-2 queue of sorted cell ids (room for every cell)
     uint32_t queue_cap;           // end - begin
     uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
+    int32_t *promo;               // [promo_cap][kPromoPlanes] plane owners of the cells queued for tier 1
+    uint32_t promo_cap;           // queue slots that keep their owners
     uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
     // batch of clouds (null clouds: one cloud described by hdr)
     const IndexHeader *clouds;    // [n_clouds] per-cloud box and subtree root
@@ -158,10 +160,14 @@ __global__ void cell_prep_kernel(uint32_t *counters) {
 }
 
 // Every tier keeps its cells in LDS: the scratch holds the work counters,
-// the row-pool counter and three per-cell lists (the FP64 tier-0 flags and
-// the tier-1 and tier-2 queues, each with room for every cell, so no queue
-// can fill).
-uint64_t cell_scratch_bytes(int64_t cells) { return 256 + 12 * (uint64_t)cells; }
+// the row-pool counter, three per-cell lists (the FP64 tier-0 flags and the
+// tier-1 and tier-2 queues, each with room for every cell, so no queue can
+// fill) and the plane owners of the first cells queued for tier 1 (room for
+// one cell in 32, and 64 more; a cell past them starts tier 1 without).
+uint64_t promo_slots(int64_t cells) { return (uint64_t)cells / 32 + 64; }
+uint64_t cell_scratch_bytes(int64_t cells) {
+    return 256 + 12 * (uint64_t)cells + 4 * (uint64_t)kPromoPlanes * promo_slots(cells);
+}
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     const IndexLayout L = make_index_layout(a.n);
@@ -207,6 +213,8 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.big_list = (uint32_t *)(sc + 256);
     prm.huge_list = (uint32_t *)(sc + 256 + 4 * count);
     prm.exact_list = (uint32_t *)(sc + 256 + 8 * count);
+    prm.promo = (int32_t *)(sc + 256 + 12 * count);
+    prm.promo_cap = (uint32_t)promo_slots(count);
     const int64_t cells_per_block = kT0Threads / kT0Lanes;
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;

@@ -60,6 +60,7 @@ enum StatSlot : int {
     kNumStats = 32,
 };
 constexpr int kExactLogSlots = 7;
+constexpr int kPromoPlanes = 48;  // plane owners kept for a cell promoted to tier 1 (tier 0's plane capacity)
 // Statistics first summed per group in LDS: slots 0..11, then the two bail
 // counters (LDS slots 12 and 13).
 constexpr int kCtrBailTest = 12, kCtrBailVertex = 13;

@@ -412,6 +412,7 @@ struct Group {
     bool dirty;   // the cell changed since the AABB was computed
     uint64_t bloom;
     bool in_guess;
+    bool bloom_all;  // every candidate whose owner hits the Bloom mask is checked for a plane already held
     // After the pre-clip: every point nearer the seed than this (squared,
     // lowered by a margin) is one of the guesses already offered.
     float guess_r2;
@@ -976,7 +977,7 @@ struct Group {
         }
         const float a2 = c.ax * c.ax + c.ay * c.ay + c.az * c.az;
         if (a2 < guess_r2) return false;  // offered by the pre-clip; the cell has only shrunk since
-        maybe_guess = guess_r2 == 0.0f || a2 < guess_r2 * 1.00003f;
+        maybe_guess = guess_r2 == 0.0f || a2 < guess_r2 * 1.00003f || (bloom_all && (bloom & bloom_bit(j)));
         c.H = a2 - dw;
         c.pH = a2 + fabsf(dw);
         c.pa1 = fabsf(c.ax) + fabsf(c.ay) + fabsf(c.az);
@@ -1325,6 +1326,7 @@ struct Group {
         const uint32_t orig = perm[cell];
         init_box((!kExact && prm.init_h) ? 2.0f * prm.init_h[orig] : 0.0f);
         in_guess = true;
+        bloom_all = false;
         guess_r2 = 0.0f;
         if (!prm.guesses && prm.seed_window > 0) {
             // Morton neighbours first: the cheap analogue of clipping by the
@@ -1358,6 +1360,32 @@ struct Group {
             // row's last, up to the few ulps of the two FP32 evaluations of
             // |p - s|^2, which the 1e-5 relative margin covers.
             if (strided && prm.guess_d2) guess_r2 = prm.guess_d2[cell] * (1.0f - 1e-5f);
+        }
+        in_guess = false;
+    }
+
+    // The owners of the planes held, in plane order (point ids in the index's
+    // order, negative for the box planes, -1 past them), for the large tier
+    // (group op).
+    __device__ __forceinline__ void store_owners(int32_t *out) const {
+#pragma unroll
+        for (int k = 0; k < kPRows; ++k) {
+            const int p = gl + G * k;
+            if (p < kPromoPlanes) out[p] = p < np ? __float_as_int(S->plx[p].y) : -1;
+        }
+    }
+
+    // Offers the planes a smaller tier held when it gave the cell up (point
+    // ids, negative for none): the cell starts from nearly all of its faces.  They
+    // join the Bloom mask, and every later candidate that hits the mask is
+    // checked against the planes held, so none is clipped twice.
+    __device__ __forceinline__ void offer_owners(const int32_t *ids, int m) {
+        if (status != kSuccess || overflow) return;
+        in_guess = true;
+        bloom_all = true;
+        for (int o = 0; o < m && status == kSuccess && !overflow; o += G) {
+            const int32_t id = o + gl < m ? ids[o + gl] : -1;
+            offer_batch(id >= 0 && (uint32_t)id != seed, (uint32_t)(id < 0 ? 0 : id));
         }
         in_guess = false;
     }
@@ -1552,7 +1580,14 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
                 const uint32_t slot = atomicAdd(&P.counters[1], 1u);
                 if (slot < P.queue_cap) P.big_list[slot] = cell;
                 else truncate_cell(P, orig);
+                g.S->tmp[0] = slot;
             }
+        }
+        // with the promotion, the owners of the planes held (the large tier
+        // starts from them); the slot comes through LDS from the group's lane 0
+        if (active && !emit && (kExactTier || !g.bail)) {
+            const uint32_t slot = g.S->tmp[0];
+            if (slot < P.promo_cap) g.store_owners(P.promo + (size_t)slot * kPromoPlanes);
         }
         if (active && P.cycles && g.gl == 0) P.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
         if (active && g.gl == 0 && !g.bail) cells++;
@@ -1647,6 +1682,7 @@ __global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellPa
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
         g.start(cell, *reread(&sprm));
+        if (k < prm.promo_cap) g.offer_owners(prm.promo + (size_t)k * kPromoPlanes, kPromoPlanes);
         g.traverse();
         const CellParams &P = *reread(&sprm);
         const uint32_t orig = P.perm[cell];
