@@ -26,10 +26,11 @@ STAT_NAMES = {
     0: "fp64_rechecks", 1: "unresolved", 2: "inconsistent", 3: "promoted", 4: "clips",
     5: "candidates", 6: "nodes", 7: "cells", 8: "asymmetric", 9: "dropped", 10: "leaves",
     11: "skipped", 12: "truncated", 13: "pool_short", 14: "pool_demand", 15: "edges", 16: "exact",
-    24: "bail_test", 25: "bail_vertex",
+    24: "bail_test", 25: "bail_vertex", 26: "tentative", 27: "tentative_wrong",
 }
 STAT_POOL_SHORT, STAT_POOL_DEMAND = 13, 14
 STAT_EXACT_LOG = 17  # 17..23: original ids + 1 of the first cells with an expansion-arithmetic test
+STAT_TENTATIVE_LOG = 28  # 28..31: original ids + 1 of the first cells rebuilt after a wrong FP32 sign
 CELL_BLOCK = 64  # cells per admission block (one wavefront)
 KNN_PRECLIP = 16  # nearest neighbours clipped first when knn > 0
 
@@ -329,6 +330,8 @@ def build_diagram(lib: HipLibrary, points: torch.Tensor, weights, knn: int = 8, 
             instrument.stats = {STAT_NAMES[k]: int(v) for k, v in enumerate(st.tolist()) if k in STAT_NAMES}
             instrument.stats["exact_cells"] = [int(v) - 1 for v in st[STAT_EXACT_LOG:STAT_EXACT_LOG + 7].tolist()
                                                if v > 0]
+            instrument.stats["rebuilt_cells"] = [int(v) - 1 for v in
+                                                 st[STAT_TENTATIVE_LOG:STAT_TENTATIVE_LOG + 4].tolist() if v > 0]
             instrument.plan = plan
         return adjacency, offsets, status
 

@@ -63,7 +63,8 @@ struct CellParams {
     unsigned long long *stats;
     uint32_t *counters;           // [0] tier-0 work, [1] tier-1 queue length, [2] tier-1 work,
                                   // [3] tier-2 queue length, [4] tier-2 work,
-                                  // [5] cells flagged for FP64 tier 0, [6] FP64 tier-0 windows
+                                  // [5] cells flagged for FP64 tier 0, [6] FP64 tier-0 windows,
+                                  // [7] removal tests recorded by tier 0 (taken by their FP32 sign)
     unsigned long long *pool_top; // row-pool allocation counter (in the scratch, zeroed per launch)
     uint32_t *big_list;           // tier-1 queue of sorted cell ids (room for every cell)
     uint32_t *huge_list;          // tier-2 queue of sorted cell ids (room for every cell)
@@ -71,6 +72,9 @@ struct CellParams {
     uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
     int32_t *promo;               // [promo_cap][kPromoPlanes] plane owners of the cells queued for tier 1
     uint32_t promo_cap;           // queue slots that keep their owners
+    uint32_t *tent;               // [tent_cap][5] tests tier 0 took by their FP32 sign: sorted cell id (or
+                                  // kNoCell), the owners of the vertex's planes, the query point | sign << 31
+    uint32_t tent_cap;
     uint32_t *cycles;             // optional [n]: per-cell cost counter (original order)
     // batch of clouds (null clouds: one cloud described by hdr)
     const IndexHeader *clouds;    // [n_clouds] per-cloud box and subtree root
@@ -162,11 +166,14 @@ __global__ void cell_prep_kernel(uint32_t *counters) {
 // Every tier keeps its cells in LDS: the scratch holds the work counters,
 // the row-pool counter, three per-cell lists (the FP64 tier-0 flags and the
 // tier-1 and tier-2 queues, each with room for every cell, so no queue can
-// fill) and the plane owners of the first cells queued for tier 1 (room for
-// one cell in 32, and 64 more; a cell past them starts tier 1 without).
+// fill), the plane owners of the first cells queued for tier 1 (room for
+// one cell in 32, and 64 more; a cell past them starts tier 1 without) and
+// the removal tests tier 0 took by their FP32 sign (room for one test per two
+// cells, and 1024 more; a cell past them goes to the FP64 twin).
 uint64_t promo_slots(int64_t cells) { return (uint64_t)cells / 32 + 64; }
+uint64_t tent_slots(int64_t cells) { return (uint64_t)cells / 2 + 1024; }
 uint64_t cell_scratch_bytes(int64_t cells) {
-    return 256 + 12 * (uint64_t)cells + 4 * (uint64_t)kPromoPlanes * promo_slots(cells);
+    return 256 + 12 * (uint64_t)cells + 4 * (uint64_t)kPromoPlanes * promo_slots(cells) + 20 * tent_slots(cells);
 }
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
@@ -215,6 +222,8 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.exact_list = (uint32_t *)(sc + 256 + 8 * count);
     prm.promo = (int32_t *)(sc + 256 + 12 * count);
     prm.promo_cap = (uint32_t)promo_slots(count);
+    prm.tent = (uint32_t *)(sc + 256 + 12 * count + 4 * kPromoPlanes * promo_slots(count));
+    prm.tent_cap = (uint32_t)tent_slots(count);
     const int64_t cells_per_block = kT0Threads / kT0Lanes;
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
     const int64_t max_blocks = (int64_t)a.waves * kWave / kT0Threads;
@@ -231,6 +240,11 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     hipLaunchKernelGGL(cell_prep_kernel, dim3(1), dim3(64), 0, stream, prm.counters);
     if (hipError_t e = fill_u32(prm.exact_list, count, 0u, stream)) return e;
     hipLaunchKernelGGL(cell_group_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0, stream, prm);
+    {
+        const uint64_t want = ((uint64_t)prm.tent_cap + 255) / 256;
+        hipLaunchKernelGGL(cell_tentative_check_kernel, dim3((unsigned)(want < 2048 ? want : 2048)), dim3(256), 0,
+                           stream, prm);
+    }
     if (prm.clouds != nullptr)
         hipLaunchKernelGGL(cell_group_exact_batch_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads),
                            0, stream, prm);

@@ -57,8 +57,16 @@ enum StatSlot : int {
     kStatExactLog = 17,      // 17..23: original ids of the first cells with such a test, plus 1
     kStatBailTest = 24,      // FP32-only tier 0 gave a cell up: a removal test its filter could not decide
     kStatBailVertex = 25,    // ... a new vertex whose FP32 orientation was not safely positive
+    kStatTentative = 26,     // removal tests tier 0 took by their FP32 sign (decided in FP64 after it)
+    kStatTentativeWrong = 27,  // ... whose FP64 decision differed: the cell was rebuilt by the FP64 twin
+    kStatTentativeLog = 28,  // 28..31: original ids of the first such cells, plus 1
     kNumStats = 32,
 };
+constexpr int kTentativeLogSlots = 4;
+// Removal tests one tier-0 cell may take by their FP32 sign (kept in LDS
+// until the cell is written out); a cell with more goes to the FP64 twin.
+constexpr int kTentMax = 4;
+constexpr uint32_t kNoCell = 0xFFFFFFFFu;  // a record slot that holds no test
 constexpr int kExactLogSlots = 7;
 constexpr int kPromoPlanes = 48;  // plane owners kept for a cell promoted to tier 1 (tier 0's plane capacity)
 // Statistics first summed per group in LDS: slots 0..11, then the two bail

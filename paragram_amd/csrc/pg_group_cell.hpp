@@ -22,6 +22,7 @@
 //   tmp[V]   uint32  clip and emission scratch
 //   pm[P/2]  uint32  (tier 0) per plane, a 16-bit mask of the lanes whose
 //                    removed triangle holds the plane; zero between clips
+//   tent[4]  4 x u32 (tier 0) removal tests taken by their FP32 sign
 // Vertices live in a pool of V slots with a group-uniform valid mask (slot
 // v = lane + G k belongs to lane v mod G), so a clip frees and refills slots
 // without moving data.
@@ -45,6 +46,10 @@ struct alignas(16) GroupLDS {
     uint32_t pm[P <= 64 ? (P + 1) / 2 : 1];
     uint32_t ctr[kNumCtr];  // diagnostics, in the order of the statistics slots
     uint32_t exact_events;  // removal tests of the current cell decided by expansion arithmetic
+    // (tier 0) removal tests taken by their FP32 sign: the owners of the
+    // vertex's three planes, and the query point with the sign taken in bit 31
+    uint32_t tent[P <= 64 ? kTentMax : 1][4];
+    uint32_t ntent;
 };
 
 // Candidates of one batch, staged for the group: (a, H) and (pH, owner).
@@ -149,15 +154,13 @@ __device__ __forceinline__ void fp64_plane(const float4 *pts, const IndexHeader 
 
 // Vertex of planes (i, j, k): N = H_i (a_j x a_k) + H_j (a_k x a_i) + H_k (a_i x a_j),
 // D = a_i . (a_j x a_k), y = N / D, with permanents pN, pD.
-template <int kB>
-__device__ __forceinline__ void fp64_vertex(const float2 *plx, const float4 *pts, const IndexHeader *hdr,
-                                            uint32_t t, double sx, double sy, double sz, double sw, double N[3],
-                                            double &D, double pN[3], double &pD) {
+// o0..o2: the owners of the three planes.
+__device__ __forceinline__ void fp64_vertex_o(int o0, int o1, int o2, const float4 *pts, const IndexHeader *hdr,
+                                              double sx, double sy, double sz, double sw, double N[3], double &D,
+                                              double pN[3], double &pD) {
     double a[3][3], H[3], pa[3][3], pH[3];
-    for (int e = 0; e < 3; ++e) {
-        const int owner = __float_as_int(plx[tri_get<kB>(t, e)].y);
-        fp64_plane(pts, hdr, owner, sx, sy, sz, sw, a[e], H[e], pa[e], pH[e]);
-    }
+    for (int e = 0; e < 3; ++e)
+        fp64_plane(pts, hdr, e == 0 ? o0 : (e == 1 ? o1 : o2), sx, sy, sz, sw, a[e], H[e], pa[e], pH[e]);
     double c[3][3], pc[3][3];
     for (int e = 0; e < 3; ++e) {
         const double *u = a[(e + 1) % 3], *v = a[(e + 2) % 3];
@@ -176,17 +179,24 @@ __device__ __forceinline__ void fp64_vertex(const float2 *plx, const float4 *pts
         pN[k] = pH[0] * pc[0][k] + pH[1] * pc[1][k] + pH[2] * pc[2][k];
     }
 }
+// The same for plane triple t of a cell's plane list.
+template <int kB>
+__device__ __forceinline__ void fp64_vertex(const float2 *plx, const float4 *pts, const IndexHeader *hdr,
+                                            uint32_t t, double sx, double sy, double sz, double sw, double N[3],
+                                            double &D, double pN[3], double &pD) {
+    fp64_vertex_o(__float_as_int(plx[tri_get<kB>(t, 0)].y), __float_as_int(plx[tri_get<kB>(t, 1)].y),
+                  __float_as_int(plx[tri_get<kB>(t, 2)].y), pts, hdr, sx, sy, sz, sw, N, D, pN, pD);
+}
 
 // The removal test decided exactly (expansion arithmetic, pg_exact.hpp) for
-// the vertex with plane triple t against the plane of point q_owner: bit 0
-// beyond (sign(P) sign(D) > 0), bit 1 capacity exceeded (no decision).  Its
-// own call site keeps the expansions' private arrays off the common path.
-template <int kB>
-__device__ __noinline__ int exact_removal(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+// the vertex of the planes of o0..o2 against the plane of point q_owner: bit
+// 0 beyond (sign(P) sign(D) > 0), bit 1 capacity exceeded (no decision).
+// Its own call site keeps the expansions' private arrays off the common path.
+__device__ __noinline__ int exact_removal(int o0, int o1, int o2, const float4 *pts, const IndexHeader *hdr,
                                           float sx, float sy, float sz, float sw, int q_owner) {
     xp::PlaneIn pl[4];
     for (int e = 0; e < 4; ++e) {
-        const int o = e < 3 ? __float_as_int(plx[tri_get<kB>(t, e)].y) : q_owner;
+        const int o = e == 0 ? o0 : (e == 1 ? o1 : (e == 2 ? o2 : q_owner));
         if (o >= 0) {
             const float4 p = pts[o];
             pl[e] = xp::PlaneIn{p.x, p.y, p.z, p.w, -1};
@@ -200,18 +210,17 @@ __device__ __noinline__ int exact_removal(const float2 *plx, const float4 *pts, 
     return sD * sP > 0 ? 1 : 0;
 }
 
-// FP64 removal decision for the vertex with plane triple t against the
+// FP64 removal decision for the vertex of the planes of o0..o2 against the
 // plane of point q_owner: beyond iff sign(P) sign(D) > 0 with
 // P = a_q . N - H_q D (D > 0 for every vertex a cell keeps).  Returns bit 0:
 // beyond, bit 1: undecided even by expansion arithmetic (capacity; the FP64
 // signs taken), bit 2: the vertex orientation D is certainly negative,
 // bit 3: decided by expansion arithmetic (the FP64 filter could not).
-template <int kB>
-__device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
-                                         float sx_, float sy_, float sz_, float sw_, int q_owner) {
+__device__ __forceinline__ int fp64_decide(int o0, int o1, int o2, const float4 *pts, const IndexHeader *hdr,
+                                           float sx_, float sy_, float sz_, float sw_, int q_owner) {
     const double sx = sx_, sy = sy_, sz = sz_, sw = sw_;
     double N[3], D, pN[3], pD;
-    fp64_vertex<kB>(plx, pts, hdr, t, sx, sy, sz, sw, N, D, pN, pD);
+    fp64_vertex_o(o0, o1, o2, pts, hdr, sx, sy, sz, sw, N, D, pN, pD);
     double a[3], H, pa[3], pH;
     fp64_plane(pts, hdr, q_owner, sx, sy, sz, sw, a, H, pa, pH);
     const double Pv = a[0] * N[0] + a[1] * N[1] + a[2] * N[2] - H * D;
@@ -224,14 +233,21 @@ __device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, c
     // vertex's planes gives the same plane: P = 0 exactly, not beyond.
     const float4 q = pts[q_owner];
     for (int e = 0; e < 3; ++e) {
-        const int o = __float_as_int(plx[tri_get<kB>(t, e)].y);
+        const int o = e == 0 ? o0 : (e == 1 ? o1 : o2);
         if (o < 0) continue;
         const float4 r = pts[o];
         if (r.x == q.x && r.y == q.y && r.z == q.z && r.w == q.w) return flags;
     }
-    const int x = exact_removal<kB>(plx, pts, hdr, t, sx_, sy_, sz_, sw_, q_owner);
+    const int x = exact_removal(o0, o1, o2, pts, hdr, sx_, sy_, sz_, sw_, q_owner);
     if (!(x & 2)) return flags | 8 | (x & 1);
     return flags | 2 | ((Pv > 0.0) == (D >= 0.0) ? 1 : 0);
+}
+// The same for plane triple t of a cell's plane list.
+template <int kB>
+__device__ __noinline__ int fp64_recheck(const float2 *plx, const float4 *pts, const IndexHeader *hdr, uint32_t t,
+                                         float sx, float sy, float sz, float sw, int q_owner) {
+    return fp64_decide(__float_as_int(plx[tri_get<kB>(t, 0)].y), __float_as_int(plx[tri_get<kB>(t, 1)].y),
+                       __float_as_int(plx[tri_get<kB>(t, 2)].y), pts, hdr, sx, sy, sz, sw, q_owner);
 }
 
 // Position and error radius of a vertex of planes t by expansion
@@ -535,7 +551,7 @@ struct Group {
                     ok = false;
                 }
             }
-        } else if (D < -eD && pD > kTiny32) {
+        } else if (D < -eD && pD > kTiny32 && (kExact || S->ntent == 0u)) {
             count(kStatInconsistent);
         }
         store_vertex(slot, pos);
@@ -570,7 +586,8 @@ struct Group {
 
     // Is vertex v strictly beyond the plane of q?  Filter: the error of
     // a~ . y~ - H~ against a . y - H is at most |a|_1 eps + 2e-6 (sum |a_k y_k| + pH).
-    // Returns 1 (beyond), 0 (not beyond) or -1 (undecided: FP64 recheck).
+    // Returns 1 (beyond), 0 (not beyond), or when undecided (FP64 recheck)
+    // -2 if the FP32 value is positive and -1 if not.
     // (with the stored w: |a|_1 w + 2e-6 pH + tiny bounds the error of the
     // FMA chain below against a . y - H)
     __device__ __forceinline__ int beyond32(int v, const Cand &q) const {
@@ -578,7 +595,7 @@ struct Group {
         const float Pv = fmaf(q.ax, p.x, fmaf(q.ay, p.y, fmaf(q.az, p.z, -q.H)));
         const float err = fmaf(q.pa1, p.w, q.c2);
         if (fabsf(Pv) > err) return Pv > 0.0f ? 1 : 0;
-        return -1;
+        return Pv > 0.0f ? -2 : -1;
     }
 
     // ---------------------------------------------------------- bounds
@@ -777,13 +794,14 @@ struct Group {
         uint64_t rem[kW];
 #pragma unroll
         for (int w = 0; w < kW; ++w) rem[w] = 0;
-        uint32_t mine = 0, unsure = 0;
+        uint32_t mine = 0, unsure = 0, lean = 0;
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
             if (!own(valid, k)) continue;
             const int b = beyond32(gl + G * k, q);
             if (b > 0) mine |= 1u << k;
             if (b < 0) unsure |= 1u << k;
+            if (b == -2) lean |= 1u << k;
         }
         if constexpr (kExact) {
             while (unsure) {  // one FP64 call site
@@ -792,9 +810,31 @@ struct Group {
                 if (recheck64(gl + G * k, q.owner)) mine |= 1u << k;
             }
         } else if (gany(unsure != 0u)) {
-            overflow = true;
-            bail = 1u;
-            return false;
+            // Undecided in FP32: take the sign of the FP32 value and record
+            // the test; a pass after this tier decides it in FP64 and sends
+            // the cell to the FP64-capable twin if the sign was wrong.  A
+            // cell with more such tests than its record holds goes there now.
+            bool ok = true;
+            while (unsure) {
+                const int k = (int)__builtin_ctz(unsure);
+                unsure &= unsure - 1u;
+                const uint32_t t = S->vtri[gl + G * k];
+                const uint32_t r = atomicAdd(&S->ntent, 1u);
+                if (r < (uint32_t)kTentMax) {
+                    S->tent[r][0] = __float_as_uint(S->plx[tri_get<kB>(t, 0)].y);
+                    S->tent[r][1] = __float_as_uint(S->plx[tri_get<kB>(t, 1)].y);
+                    S->tent[r][2] = __float_as_uint(S->plx[tri_get<kB>(t, 2)].y);
+                    S->tent[r][3] = (uint32_t)q.owner | (((lean >> k) & 1u) << 31);
+                } else {
+                    ok = false;
+                }
+            }
+            if (gany(!ok)) {
+                overflow = true;
+                bail = 1u;
+                return false;
+            }
+            mine |= lean;
         }
         if (!gany(mine != 0u)) return false;  // no vertex beyond: the plane does not cut
 #pragma unroll
@@ -888,7 +928,8 @@ struct Group {
         // every plane of the boundary cycle starts one edge and ends one
         if (nb < 3 || gsum(sx_) != gsum(sy_)) {
             status = kInconsistentBoundary;
-            if (gl == 0) count(kStatInconsistent);
+            // (after a test taken by its FP32 sign the cell is rebuilt instead)
+            if (gl == 0 && (kExact || S->ntent == 0u)) count(kStatInconsistent);
             return true;
         }
         uint64_t freem[kW];
@@ -1317,6 +1358,8 @@ struct Group {
         seed = cell;
         if constexpr (kExact) {
             if (gl == 0) S->exact_events = 0u;
+        } else {
+            if (gl == 0) S->ntent = 0u;
         }
         const float4 s = pts[cell];
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
@@ -1541,6 +1584,33 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
         // a cell built inside a predicted cube stands only if the cube cut nothing
         if (active && !kExactTier && !g.overflow && g.cube && (g.status != kSuccess || g.cube_touched()))
             g.overflow = true, g.bail = 3u;
+        if constexpr (!kExactTier) {
+            // the removal tests taken by their FP32 sign go to the record for
+            // the FP64 check after this tier, with the cell; a cell that ended
+            // inconsistent after such a test, or that the record cannot hold,
+            // goes to the FP64 twin instead
+            const uint32_t nt = active && !g.overflow ? g.S->ntent : 0u;
+            if (nt > 0u && g.status == kInconsistentBoundary) {
+                g.overflow = true, g.bail = 1u;
+            } else if (nt > 0u) {
+                // (the base comes from the group's lane 0 through a shuffle:
+                // a store and load of one LDS word by different lanes are not
+                // ordered for the compiler)
+                uint32_t base = 0u;
+                if (g.gl == 0) base = atomicAdd(&P.counters[7], nt);
+                base = (uint32_t)g.gbi((int)base, 0);
+                const bool fits = (uint64_t)base + nt <= (uint64_t)P.tent_cap;
+                if ((uint32_t)g.gl < nt && (uint64_t)base + g.gl < (uint64_t)P.tent_cap) {
+                    uint32_t *rec = P.tent + 5 * ((uint64_t)base + g.gl);
+                    rec[0] = fits ? cell : kNoCell;
+                    rec[1] = g.S->tent[g.gl][0];
+                    rec[2] = g.S->tent[g.gl][1];
+                    rec[3] = g.S->tent[g.gl][2];
+                    rec[4] = g.S->tent[g.gl][3];
+                }
+                if (!fits) g.overflow = true, g.bail = 1u;
+            }
+        }
         const bool emit = active && !g.overflow;
         const int F = emit ? g.stage_row(P) : 0;
         // the wave's four rows, back to back in the wave's current pool chunk
@@ -1570,23 +1640,24 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
             } else if (g.gl == 0) {
                 pool_short(P, cell, orig, base + (uint64_t)F);
             }
-        } else if (active && g.gl == 0) {
+        }
+        uint32_t slot = 0u;
+        if (!emit && active && g.gl == 0) {
             if (!kExactTier && g.bail) {  // flag for the FP64-capable instance
                 if (g.bail < 3u) g.count(g.bail == 1u ? kCtrBailTest : kCtrBailVertex);
                 P.exact_list[c] = 1u;
                 atomicAdd(&P.counters[5], 1u);
             } else {  // queue for the large-capacity tier
                 g.count(kStatPromoted);
-                const uint32_t slot = atomicAdd(&P.counters[1], 1u);
+                slot = atomicAdd(&P.counters[1], 1u);
                 if (slot < P.queue_cap) P.big_list[slot] = cell;
                 else truncate_cell(P, orig);
-                g.S->tmp[0] = slot;
             }
         }
         // with the promotion, the owners of the planes held (the large tier
-        // starts from them); the slot comes through LDS from the group's lane 0
+        // starts from them); the slot comes from the group's lane 0
         if (active && !emit && (kExactTier || !g.bail)) {
-            const uint32_t slot = g.S->tmp[0];
+            slot = (uint32_t)g.gbi((int)slot, 0);
             if (slot < P.promo_cap) g.store_owners(P.promo + (size_t)slot * kPromoPlanes);
         }
         if (active && P.cycles && g.gl == 0) P.cycles[orig] = (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
@@ -1659,6 +1730,45 @@ __global__ void __launch_bounds__(kT0Threads) cell_group_exact_kernel(const Cell
 __global__ void __launch_bounds__(kT0Threads) __attribute__((amdgpu_waves_per_eu(3, 3)))
 cell_group_exact_batch_kernel(const CellParams prm) {
     tier0_body<Tier0x, true>(prm);
+}
+
+// Decides in FP64 (where FP64 cannot, exactly) the removal tests tier 0 took
+// by their FP32 sign, and flags the cell of every test whose sign was wrong
+// for the FP64-capable twin, which rebuilds the cell (its row, count and
+// status replace tier 0's).  One thread per recorded test.
+__global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellParams prm) {
+    const uint32_t m = min(prm.counters[7], prm.tent_cap);
+    uint32_t tests = 0, unresolved = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+        const uint32_t *rec = prm.tent + 5 * (uint64_t)r;
+        const uint32_t cell = rec[0];
+        if (cell == kNoCell) continue;
+        const IndexHeader *hdr =
+            prm.clouds != nullptr ? &prm.clouds[batch_find(prm.cloud_cells, prm.n_clouds, cell)] : prm.hdr;
+        const float4 s = prm.pts[cell];
+        const uint32_t qv = rec[4];
+        const int res = fp64_decide((int)rec[1], (int)rec[2], (int)rec[3], prm.pts, hdr, s.x, s.y, s.z, s.w,
+                                    (int)(qv & 0x7FFFFFFFu));
+        ++tests;
+        if (res & 2) ++unresolved;
+        if (res & 8) log_exact(prm, prm.perm[cell], 1u);
+        if ((uint32_t)(res & 1) != (qv >> 31)) {
+            prm.exact_list[cell - prm.begin] = 1u;
+            atomicAdd(&prm.counters[5], 1u);
+            const unsigned long long k = atomicAdd(&prm.stats[kStatTentativeWrong], 1ull);
+            if (k < (unsigned long long)kTentativeLogSlots)
+                prm.stats[kStatTentativeLog + k] = (unsigned long long)prm.perm[cell] + 1ull;
+        }
+    }
+    tests = wave_sum(tests);
+    unresolved = wave_sum(unresolved);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (tests) {
+            atomicAdd(&prm.stats[kStatTentative], (unsigned long long)tests);
+            atomicAdd(&prm.stats[kStatFp64Rechecks], (unsigned long long)tests);
+        }
+        if (unresolved) atomicAdd(&prm.stats[kStatUnresolved], (unsigned long long)unresolved);
+    }
 }
 
 // Tier 1: one cell per wave, the queued cells.

@@ -5,7 +5,10 @@ Checked against the oracle: a seeded sample of 400 cells of every cloud and
 every cell of the four largest clouds (cloud-local ids, rows and statuses
 bit-exact); CSR invariants of every cloud; no cell truncated (status 1) and
 no row-pool shortfall; every cell whose removal tests needed expansion
-arithmetic checked against the oracle as well.
+arithmetic checked against the oracle as well.  Tier 0 takes the removal
+tests its FP32 filter leaves open by their FP32 sign and the FP64 check
+after it sends the cells of wrong signs to the FP64 twin: both happen here,
+and the first cells rebuilt that way are checked against the oracle too.
 """
 import concurrent.futures
 
@@ -45,6 +48,8 @@ def test_c4_256_lidar_clouds_one_launch(gpu):
     assert st["pool_short"] == 0
     assert st["unresolved"] == 0, "a removal test was left to the FP64 sign"
     assert st["asymmetric"] == 0 and st["dropped"] == 0
+    assert st["tentative"] > 0 and 0 < st["tentative_wrong"] < st["tentative"]
+    assert st["rebuilt_cells"], "the first rebuilt cells are logged"
 
     log("C4: CSR invariants of every cloud")
     with concurrent.futures.ThreadPoolExecutor(16) as ex:
@@ -52,9 +57,10 @@ def test_c4_256_lidar_clouds_one_launch(gpu):
 
     sizes = off[1:] - off[:-1]
     full = set(np.argsort(sizes)[-4:].tolist())
-    exact_cells = st["exact_cells"]
+    exact_cells = st["exact_cells"] + st["rebuilt_cells"]
     log(f"C4: oracle on 400 cells per cloud and every cell of clouds {sorted(full)}; "
-        f"cells with expansion-arithmetic tests: {exact_cells}")
+        f"cells with expansion-arithmetic tests, then the first cells rebuilt after a wrong FP32 sign: "
+        f"{exact_cells}")
     bad = []
     checked = 0
     for c, cloud in enumerate(clouds):
