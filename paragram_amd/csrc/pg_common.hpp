@@ -65,7 +65,7 @@ enum StatSlot : int {
 constexpr int kTentativeLogSlots = 4;
 // Removal tests one tier-0 cell may take by their FP32 sign (kept in LDS
 // until the cell is written out); a cell with more goes to the FP64 twin.
-constexpr int kTentMax = 4;
+constexpr int kTentMax = 8;
 constexpr uint32_t kNoCell = 0xFFFFFFFFu;  // a record slot that holds no test
 constexpr int kExactLogSlots = 7;
 constexpr int kPromoPlanes = 48;  // plane owners kept for a cell promoted to tier 1 (tier 0's plane capacity)

@@ -22,7 +22,7 @@
 //   tmp[V]   uint32  clip and emission scratch
 //   pm[P/2]  uint32  (tier 0) per plane, a 16-bit mask of the lanes whose
 //                    removed triangle holds the plane; zero between clips
-//   tent[4]  4 x u32 (tier 0) removal tests taken by their FP32 sign
+//   tent[8]  2 x u32 (tier 0) removal tests taken by their FP32 sign
 // Vertices live in a pool of V slots with a group-uniform valid mask (slot
 // v = lane + G k belongs to lane v mod G), so a clip frees and refills slots
 // without moving data.
@@ -46,9 +46,12 @@ struct alignas(16) GroupLDS {
     uint32_t pm[P <= 64 ? (P + 1) / 2 : 1];
     uint32_t ctr[kNumCtr];  // diagnostics, in the order of the statistics slots
     uint32_t exact_events;  // removal tests of the current cell decided by expansion arithmetic
-    // (tier 0) removal tests taken by their FP32 sign: the owners of the
-    // vertex's three planes, and the query point with the sign taken in bit 31
-    uint32_t tent[P <= 64 ? kTentMax : 1][4];
+    // (tier 0) removal tests taken by their FP32 sign: the vertex's plane
+    // triple (slots, kept and renumbered by the plane collection), and the
+    // query point with the sign taken in bit 31
+    uint32_t tent[P <= 64 ? kTentMax : 1][2];
+    // their number (every lane of the group writes it, with the same value,
+    // so each lane reads back its own store)
     uint32_t ntent;
 };
 
@@ -743,10 +746,24 @@ struct Group {
         return (uint32_t)r;
     }
 
-    // Drops unreferenced planes and renumbers the vertex triples.
+    // Drops unreferenced planes and renumbers the vertex triples (the planes
+    // of a recorded test's vertex stay, renumbered too, until the record is
+    // written out with the cell).
     __device__ __forceinline__ void garbage_collect() {
         uint64_t used[kPW];
         used_planes(used);
+        const uint32_t nt = kExact ? 0u : S->ntent;
+        if constexpr (!kExact) {
+            for (uint32_t i = 0; i < nt; ++i) {
+                const uint32_t t = S->tent[i][0];
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const uint32_t p = tri_get<kB>(t, e);
+#pragma unroll
+                    for (int w = 0; w < kPW; ++w) used[w] |= (int)(p >> 6) == w ? (1ull << (p & 63)) : 0ull;
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < kPRows; ++k) {
             const int p = gl + G * k;
@@ -772,6 +789,13 @@ struct Group {
                 const uint32_t t = S->vtri[v];
                 S->vtri[v] = tri_pack<kB>(plane_rank(used, tri_get<kB>(t, 0)), plane_rank(used, tri_get<kB>(t, 1)),
                                          plane_rank(used, tri_get<kB>(t, 2)));
+            }
+        }
+        if constexpr (!kExact) {
+            if ((uint32_t)gl < nt) {
+                const uint32_t t = S->tent[gl][0];
+                S->tent[gl][0] = tri_pack<kB>(plane_rank(used, tri_get<kB>(t, 0)),
+                                              plane_rank(used, tri_get<kB>(t, 1)), plane_rank(used, tri_get<kB>(t, 2)));
             }
         }
         np = mask_popc(used);
@@ -814,26 +838,22 @@ struct Group {
             // the test; a pass after this tier decides it in FP64 and sends
             // the cell to the FP64-capable twin if the sign was wrong.  A
             // cell with more such tests than its record holds goes there now.
-            bool ok = true;
-            while (unsure) {
-                const int k = (int)__builtin_ctz(unsure);
-                unsure &= unsure - 1u;
-                const uint32_t t = S->vtri[gl + G * k];
-                const uint32_t r = atomicAdd(&S->ntent, 1u);
-                if (r < (uint32_t)kTentMax) {
-                    S->tent[r][0] = __float_as_uint(S->plx[tri_get<kB>(t, 0)].y);
-                    S->tent[r][1] = __float_as_uint(S->plx[tri_get<kB>(t, 1)].y);
-                    S->tent[r][2] = __float_as_uint(S->plx[tri_get<kB>(t, 2)].y);
-                    S->tent[r][3] = (uint32_t)q.owner | (((lean >> k) & 1u) << 31);
-                } else {
-                    ok = false;
-                }
-            }
-            if (gany(!ok)) {
+            const uint32_t n0 = S->ntent;
+            int tot;
+            uint32_t r = n0 + (uint32_t)gexcl(__popc(unsure), tot);
+            if (n0 + (uint32_t)tot > (uint32_t)kTentMax) {
                 overflow = true;
                 bail = 1u;
                 return false;
             }
+            while (unsure) {
+                const int k = (int)__builtin_ctz(unsure);
+                unsure &= unsure - 1u;
+                S->tent[r][0] = S->vtri[gl + G * k];
+                S->tent[r][1] = (uint32_t)q.owner | (((lean >> k) & 1u) << 31);
+                ++r;
+            }
+            S->ntent = n0 + (uint32_t)tot;
             mine |= lean;
         }
         if (!gany(mine != 0u)) return false;  // no vertex beyond: the plane does not cut
@@ -1359,7 +1379,7 @@ struct Group {
         if constexpr (kExact) {
             if (gl == 0) S->exact_events = 0u;
         } else {
-            if (gl == 0) S->ntent = 0u;
+            S->ntent = 0u;
         }
         const float4 s = pts[cell];
         sx = s.x, sy = s.y, sz = s.z, sw = s.w;
@@ -1602,11 +1622,12 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
                 const bool fits = (uint64_t)base + nt <= (uint64_t)P.tent_cap;
                 if ((uint32_t)g.gl < nt && (uint64_t)base + g.gl < (uint64_t)P.tent_cap) {
                     uint32_t *rec = P.tent + 5 * ((uint64_t)base + g.gl);
+                    const uint32_t t = g.S->tent[g.gl][0];
                     rec[0] = fits ? cell : kNoCell;
-                    rec[1] = g.S->tent[g.gl][0];
-                    rec[2] = g.S->tent[g.gl][1];
-                    rec[3] = g.S->tent[g.gl][2];
-                    rec[4] = g.S->tent[g.gl][3];
+                    rec[1] = __float_as_uint(g.S->plx[tri_get<T::kB>(t, 0)].y);
+                    rec[2] = __float_as_uint(g.S->plx[tri_get<T::kB>(t, 1)].y);
+                    rec[3] = __float_as_uint(g.S->plx[tri_get<T::kB>(t, 2)].y);
+                    rec[4] = g.S->tent[g.gl][1];
                 }
                 if (!fits) g.overflow = true, g.bail = 1u;
             }
