@@ -51,8 +51,9 @@ WORKLOADS = {
            "{total:,} points in all), seed {seed}, one launch"),
 }
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
-CELL_KERNELS = ("cell_group_kernel", "cell_group_exact_kernel", "cell_group_exact_batch_kernel",
-                "cell_group_big_kernel", "cell_group_huge_kernel")  # (the FP64 twin's batch form launches for batches)
+CELL_KERNELS = ("cell_group_kernel", "cell_tentative_check_kernel", "cell_group_exact_kernel",
+                "cell_group_exact_batch_kernel", "cell_group_big_kernel",
+                "cell_group_huge_kernel")  # (the FP64 twin's batch form launches for batches)
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
@@ -114,7 +115,8 @@ def summarize_pmc(fetch_dir: str, write_dir: str, copy_mib: int, workload_key: s
     return rec
 
 
-STAT_KEYS = ("fp64_rechecks", "exact", "unresolved", "promoted", "truncated", "pool_short", "edges")
+STAT_KEYS = ("fp64_rechecks", "exact", "unresolved", "promoted", "truncated", "pool_short", "edges", "tentative",
+             "tentative_wrong", "bail_test")
 
 
 def _progress(msg: str) -> None:

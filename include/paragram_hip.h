@@ -132,7 +132,8 @@ int pg_row_pool_elems(int64_t cells, uint64_t *elems);
  * changes speed only, never the result).
  * stats: PG_NUM_STATS counters, accumulated (stats[26]: removal tests tier 0
  * took by their FP32 sign, all decided in FP64 after it; stats[27]: those
- * whose sign was wrong, their cells rebuilt by the FP64-capable instance;
+ * whose sign was wrong (or whose vertex lay on a plane of a predicted cube),
+ * their cells rebuilt by the FP64-capable instance;
  * stats[28..31]: original ids + 1 of the first such cells).
  * cell_cycles: optional (may be null) per-cell cost in GPU clock cycles,
  * original order; bit 31 set marks cells built by a large-capacity tier.

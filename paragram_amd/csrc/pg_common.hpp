@@ -58,7 +58,8 @@ enum StatSlot : int {
     kStatBailTest = 24,      // FP32-only tier 0 gave a cell up: a removal test its filter could not decide
     kStatBailVertex = 25,    // ... a new vertex whose FP32 orientation was not safely positive
     kStatTentative = 26,     // removal tests tier 0 took by their FP32 sign (decided in FP64 after it)
-    kStatTentativeWrong = 27,  // ... whose FP64 decision differed: the cell was rebuilt by the FP64 twin
+    kStatTentativeWrong = 27,  // ... whose FP64 decision differed (or whose vertex lay on a predicted
+                               // cube's plane): the cell was rebuilt by the FP64 twin
     kStatTentativeLog = 28,  // 28..31: original ids of the first such cells, plus 1
     kNumStats = 32,
 };

@@ -1756,10 +1756,13 @@ cell_group_exact_batch_kernel(const CellParams prm) {
 // Decides in FP64 (where FP64 cannot, exactly) the removal tests tier 0 took
 // by their FP32 sign, and flags the cell of every test whose sign was wrong
 // for the FP64-capable twin, which rebuilds the cell (its row, count and
-// status replace tier 0's).  One thread per recorded test.
+// status replace tier 0's).  A test whose vertex lies on a plane of a
+// predicted cube (owner codes -7..-12, not a plane of the diagram) is not
+// decided: its cell is flagged as well, and the twin builds it without the
+// cube.  One thread per recorded test.
 __global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellParams prm) {
     const uint32_t m = min(prm.counters[7], prm.tent_cap);
-    uint32_t tests = 0, unresolved = 0;
+    uint32_t tests = 0, decided = 0, unresolved = 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
         const uint32_t *rec = prm.tent + 5 * (uint64_t)r;
         const uint32_t cell = rec[0];
@@ -1768,12 +1771,17 @@ __global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellPar
             prm.clouds != nullptr ? &prm.clouds[batch_find(prm.cloud_cells, prm.n_clouds, cell)] : prm.hdr;
         const float4 s = prm.pts[cell];
         const uint32_t qv = rec[4];
-        const int res = fp64_decide((int)rec[1], (int)rec[2], (int)rec[3], prm.pts, hdr, s.x, s.y, s.z, s.w,
-                                    (int)(qv & 0x7FFFFFFFu));
+        const int o0 = (int)rec[1], o1 = (int)rec[2], o2 = (int)rec[3];
+        bool wrong = true;
         ++tests;
-        if (res & 2) ++unresolved;
-        if (res & 8) log_exact(prm, prm.perm[cell], 1u);
-        if ((uint32_t)(res & 1) != (qv >> 31)) {
+        if (o0 >= -6 && o1 >= -6 && o2 >= -6) {
+            const int res = fp64_decide(o0, o1, o2, prm.pts, hdr, s.x, s.y, s.z, s.w, (int)(qv & 0x7FFFFFFFu));
+            ++decided;
+            if (res & 2) ++unresolved;
+            if (res & 8) log_exact(prm, prm.perm[cell], 1u);
+            wrong = (uint32_t)(res & 1) != (qv >> 31);
+        }
+        if (wrong) {
             prm.exact_list[cell - prm.begin] = 1u;
             atomicAdd(&prm.counters[5], 1u);
             const unsigned long long k = atomicAdd(&prm.stats[kStatTentativeWrong], 1ull);
@@ -1782,11 +1790,12 @@ __global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellPar
         }
     }
     tests = wave_sum(tests);
+    decided = wave_sum(decided);
     unresolved = wave_sum(unresolved);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (tests) {
             atomicAdd(&prm.stats[kStatTentative], (unsigned long long)tests);
-            atomicAdd(&prm.stats[kStatFp64Rechecks], (unsigned long long)tests);
+            if (decided) atomicAdd(&prm.stats[kStatFp64Rechecks], (unsigned long long)decided);
         }
         if (unresolved) atomicAdd(&prm.stats[kStatUnresolved], (unsigned long long)unresolved);
     }
