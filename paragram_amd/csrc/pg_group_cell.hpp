@@ -1585,6 +1585,25 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     const int lane = threadIdx.x & (kWave - 1);
     const int grp = lane / kT0Lanes;
     if (kExactTier && prm.counters[5] == 0u) return;  // no cell was flagged
+    if (kExactTier && prm.counters[5] <= kFewFlagged) {
+        // a few flagged cells: queued for tier 1, with no plane owners
+        const int64_t count = prm.end - prm.begin;
+        while (true) {
+            uint32_t w = 0;
+            if (lane == 0) w = atomicAdd(&prm.counters[6], 1u);
+            const int64_t base = (int64_t)uniform(w) * kWave;
+            if (base >= count) break;
+            if (base + lane < count && prm.exact_list[base + lane] != 0u) {
+                const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
+                const uint32_t cell = (uint32_t)(prm.begin + base + lane);
+                if (slot < prm.queue_cap) prm.big_list[slot] = cell;
+                else truncate_cell(prm, prm.perm[cell]);
+                if (slot < prm.promo_cap)
+                    for (int p = 0; p < kPromoPlanes; ++p) prm.promo[(size_t)slot * kPromoPlanes + p] = -1;
+            }
+        }
+        return;
+    }
     T g;
     g.setup(prm, &lds[(threadIdx.x / kWave) * kCellsPerWave + grp], &cands[(threadIdx.x / kWave) * kCellsPerWave + grp]);
     uint32_t cells = 0;
