@@ -1586,16 +1586,14 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     const int grp = lane / kT0Lanes;
     if (kExactTier && prm.counters[5] == 0u) return;  // no cell was flagged
     if (kExactTier && prm.counters[5] <= kFewFlagged) {
-        // a few flagged cells: queued for tier 1, with no plane owners
+        // a few flagged cells: queued for tier 1, with no plane owners (the
+        // flags read in a grid-stride loop: no work counter to contend for)
         const int64_t count = prm.end - prm.begin;
-        while (true) {
-            uint32_t w = 0;
-            if (lane == 0) w = atomicAdd(&prm.counters[6], 1u);
-            const int64_t base = (int64_t)uniform(w) * kWave;
-            if (base >= count) break;
-            if (base + lane < count && prm.exact_list[base + lane] != 0u) {
+        const int64_t step = (int64_t)gridDim.x * kT0Threads;
+        for (int64_t c = (int64_t)blockIdx.x * kT0Threads + threadIdx.x; c < count; c += step) {
+            if (prm.exact_list[c] != 0u) {
                 const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
-                const uint32_t cell = (uint32_t)(prm.begin + base + lane);
+                const uint32_t cell = (uint32_t)(prm.begin + c);
                 if (slot < prm.queue_cap) prm.big_list[slot] = cell;
                 else truncate_cell(prm, prm.perm[cell]);
                 if (slot < prm.promo_cap)
