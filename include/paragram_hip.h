@@ -103,10 +103,10 @@ int pg_default_waves(int32_t *waves);
 
 /* Bytes of cell scratch for launches over at most max_cells cells (work
  * counters, the row-pool counter, per cell the FP64 flag and the queues of
- * the two large-capacity tiers: 12 bytes, the 48 plane owners of up to
- * max_cells / 32 + 64 cells promoted to tier 1, and 20-byte records of up
+ * the three large-capacity tiers: 16 bytes, the 48 plane owners of up to
+ * max_cells / 32 + 64 cells promoted from tier 0, and 20-byte records of up
  * to max_cells / 2 + 1024 removal tests that tier 0 took by their FP32 sign
- * for the FP64 check after it: 28 bytes a cell in all; the cells themselves
+ * for the FP64 check after it: 32 bytes a cell in all; the cells themselves
  * live in LDS).
  * Replaces ConvexCellMemory::required_bytes (voronoi_convex_cell.cuh:37-44). */
 int pg_cell_scratch_bytes(int64_t max_cells, uint64_t *bytes);
@@ -145,11 +145,11 @@ int pg_compute_cells(int64_t n, const void *workspace, int64_t begin, int64_t en
                      void *scratch, uint64_t scratch_bytes, int32_t waves, uint32_t *cell_cycles, void *stream);
 
 /* Largest cell range (end - begin) of pg_compute_cells_noalloc:
- * 1,496,791 cells (its static cell scratch holds 40 MiB, 28 bytes a cell). */
+ * 1,571,837 cells (its static cell scratch holds 48 MiB, 32 bytes a cell). */
 int pg_noalloc_max_cells(int64_t *cells);
 
 /* pg_compute_cells_ex without the scratch arguments: the cell scratch is a
- * static 40 MiB array of the library on each GPU (held until the process
+ * static 48 MiB array of the library on each GPU (held until the process
  * ends), so the call allocates nothing (no hipMalloc, hipFree or allocating
  * library routine); the caller still passes the inputs and outputs.  Ranges
  * of more than pg_noalloc_max_cells cells are refused (split them).  Calls

@@ -13,7 +13,7 @@ Same environment semantics as the reference's plan_convex_cell_chunks
 The admission unit here is a block of 64 Morton-consecutive cells (one
 wavefront) and the chunked resource is the row pool of those cells (32
 entries per cell by default), the larger of the two per-cell buffers (the
-cell scratch takes 28 bytes per cell).
+cell scratch takes 32 bytes per cell).
 """
 from __future__ import annotations
 

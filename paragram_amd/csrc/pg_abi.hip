@@ -37,7 +37,7 @@ int check_n(int64_t n) {
 // the process ends.  In host code `scratch` names a host-side stand-in, not
 // the device array: its device address comes from hipGetSymbolAddress, once
 // per device, and only that address is handed to kernels.
-constexpr uint64_t kNoAllocScratchBytes = 40ull << 20;
+constexpr uint64_t kNoAllocScratchBytes = 48ull << 20;
 __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -585,7 +585,7 @@ int pg_fill_u32(uint32_t *p, int64_t n, uint32_t v, void *stream) {
 // The most cells one launch of pg_compute_cells_noalloc covers: the largest
 // range whose cell scratch fits the static array.
 int64_t noalloc_max_cells() {
-    int64_t c = (int64_t)(kNoAllocScratchBytes / 28);
+    int64_t c = (int64_t)(kNoAllocScratchBytes / 32);
     while (c > 0 && pg::cell_scratch_bytes(c) > kNoAllocScratchBytes) --c;
     while (pg::cell_scratch_bytes(c + 1) <= kNoAllocScratchBytes) ++c;
     return c;

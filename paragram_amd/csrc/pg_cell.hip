@@ -61,16 +61,18 @@ struct CellParams {
     int32_t *counts;
     int32_t *status;
     unsigned long long *stats;
-    uint32_t *counters;           // [0] tier-0 work, [1] tier-1 queue length, [2] tier-1 work,
+    uint32_t *counters;           // [0] tier-0 work, [1] tier-m queue length, [2] tier-m work,
                                   // [3] tier-2 queue length, [4] tier-2 work,
                                   // [5] cells flagged for FP64 tier 0, [6] FP64 tier-0 windows,
-                                  // [7] removal tests recorded by tier 0 (taken by their FP32 sign)
+                                  // [7] removal tests recorded by tier 0 (taken by their FP32 sign),
+                                  // [8] tier-1 queue length, [9] tier-1 work
     unsigned long long *pool_top; // row-pool allocation counter (in the scratch, zeroed per launch)
+    uint32_t *mid_list;           // tier-m queue of sorted cell ids (room for every cell)
     uint32_t *big_list;           // tier-1 queue of sorted cell ids (room for every cell)
     uint32_t *huge_list;          // tier-2 queue of sorted cell ids (room for every cell)
     uint32_t queue_cap;           // end - begin
     uint32_t *exact_list;         // [end - begin] flags: cell for the FP64 tier-0 instance
-    int32_t *promo;               // [promo_cap][kPromoPlanes] plane owners of the cells queued for tier 1
+    int32_t *promo;               // [promo_cap][kPromoPlanes] plane owners of the cells queued for tier m
     uint32_t promo_cap;           // queue slots that keep their owners
     uint32_t *tent;               // [tent_cap][5] tests tier 0 took by their FP32 sign: sorted cell id (or
                                   // kNoCell), the owners of the vertex's planes, the query point | sign << 31
@@ -164,16 +166,16 @@ __global__ void cell_prep_kernel(uint32_t *counters) {
 }
 
 // Every tier keeps its cells in LDS: the scratch holds the work counters,
-// the row-pool counter, three per-cell lists (the FP64 tier-0 flags and the
-// tier-1 and tier-2 queues, each with room for every cell, so no queue can
-// fill), the plane owners of the first cells queued for tier 1 (room for
-// one cell in 32, and 64 more; a cell past them starts tier 1 without) and
+// the row-pool counter, four per-cell lists (the FP64 tier-0 flags and the
+// tier-m, tier-1 and tier-2 queues, each with room for every cell, so no
+// queue can fill), the plane owners of the first cells queued for tier m
+// (room for one cell in 32, and 64 more; a cell past them starts without) and
 // the removal tests tier 0 took by their FP32 sign (room for one test per two
 // cells, and 1024 more; a cell past them goes to the FP64 twin).
 uint64_t promo_slots(int64_t cells) { return (uint64_t)cells / 32 + 64; }
 uint64_t tent_slots(int64_t cells) { return (uint64_t)cells / 2 + 1024; }
 uint64_t cell_scratch_bytes(int64_t cells) {
-    return 256 + 12 * (uint64_t)cells + 4 * (uint64_t)kPromoPlanes * promo_slots(cells) + 20 * tent_slots(cells);
+    return 256 + 16 * (uint64_t)cells + 4 * (uint64_t)kPromoPlanes * promo_slots(cells) + 20 * tent_slots(cells);
 }
 
 hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
@@ -220,9 +222,10 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     prm.big_list = (uint32_t *)(sc + 256);
     prm.huge_list = (uint32_t *)(sc + 256 + 4 * count);
     prm.exact_list = (uint32_t *)(sc + 256 + 8 * count);
-    prm.promo = (int32_t *)(sc + 256 + 12 * count);
+    prm.mid_list = (uint32_t *)(sc + 256 + 12 * count);
+    prm.promo = (int32_t *)(sc + 256 + 16 * count);
     prm.promo_cap = (uint32_t)promo_slots(count);
-    prm.tent = (uint32_t *)(sc + 256 + 12 * count + 4 * kPromoPlanes * promo_slots(count));
+    prm.tent = (uint32_t *)(sc + 256 + 16 * count + 4 * kPromoPlanes * promo_slots(count));
     prm.tent_cap = (uint32_t)tent_slots(count);
     const int64_t cells_per_block = kT0Threads / kT0Lanes;
     const int64_t blocks_needed = (count + cells_per_block - 1) / cells_per_block;
@@ -251,6 +254,9 @@ hipError_t launch_cells(const CellArgs &a, hipStream_t stream) {
     else
         hipLaunchKernelGGL(cell_group_exact_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kT0Threads), 0,
                            stream, prm);
+    // tier m keeps 12 waves per CU resident where tier 1 keeps 8
+    hipLaunchKernelGGL(cell_group_mid_kernel, dim3((unsigned)(a.waves1 * 3 / 2 * kWave / kTmThreads)),
+                       dim3(kTmThreads), 0, stream, prm);
     hipLaunchKernelGGL(cell_group_big_kernel, dim3((unsigned)(a.waves1 * kWave / kT1Threads)), dim3(kT1Threads), 0,
                        stream, prm);
     hipLaunchKernelGGL(cell_group_huge_kernel, dim3((unsigned)kTier2Waves), dim3(kT2Threads), 0, stream, prm);

@@ -68,12 +68,12 @@ constexpr int kTentativeLogSlots = 4;
 // until the cell is written out); a cell with more goes to the FP64 twin.
 constexpr int kTentMax = 8;
 constexpr uint32_t kNoCell = 0xFFFFFFFFu;  // a record slot that holds no test
-// Up to this many cells flagged for the FP64-capable tier 0 go to tier 1's
-// queue instead (tier 1 builds them beside its own; the twin's launch would
+// Up to this many cells flagged for the FP64-capable tier 0 go to tier m's
+// queue instead (tier m builds them beside its own; the twin's launch would
 // last as long as its slowest cell for a handful of cells).
 constexpr uint32_t kFewFlagged = 2048;
 constexpr int kExactLogSlots = 7;
-constexpr int kPromoPlanes = 48;  // plane owners kept for a cell promoted to tier 1 (tier 0's plane capacity)
+constexpr int kPromoPlanes = 48;  // plane owners kept for a cell promoted to tier m (tier 0's plane capacity)
 // Statistics first summed per group in LDS: slots 0..11, then the two bail
 // counters (LDS slots 12 and 13).
 constexpr int kCtrBailTest = 12, kCtrBailVertex = 13;

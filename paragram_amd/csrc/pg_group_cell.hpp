@@ -2,12 +2,15 @@
 // whole cell in LDS.  Included by pg_cell.hip (it uses CellParams, the
 // precision constants and the plane-triple helpers defined there).
 //
-// Three instances (capacity tiers):
-//   tier 0: G = 16 lanes, 64 vertex slots, 48 plane slots (2,688 B of LDS per
-//           cell, 4 cells per wave) -- every cell starts here;
+// Four instances (capacity tiers):
+//   tier 0: G = 16 lanes, 64 vertex slots, 48 plane slots (about 3.3 KB of
+//           LDS per cell, 4 cells per wave) -- every cell starts here;
+//   tier m: G = 64 lanes, 256 vertex slots, 192 plane slots (one cell per
+//           wave, 12 waves per CU) -- the few cells that outgrow tier 0
+//           (cells with more than about 34 faces);
 //   tier 1: G = 64 lanes, 512 vertex slots, 255 plane slots (one cell per
-//           wave) -- the few cells that outgrow tier 0 (cells with more than
-//           about 34 faces, e.g. near-range LiDAR returns with 200+ faces);
+//           wave, 8 waves per CU) -- the cells that outgrow tier m (e.g.
+//           near-range LiDAR returns with 200+ faces);
 //   tier 2: G = 64 lanes, 1024 vertex slots, 1023 plane slots, 10-bit plane
 //           indices (one cell per wave and block) -- the rare cells that
 //           outgrow tier 1.
@@ -1510,10 +1513,12 @@ __device__ __forceinline__ int batch_find(const int64_t *off, int n_clouds, int6
 constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Threads = 256;
 // (a depth-first stack holds at most one entry per level of the hierarchy,
 // fewer than 64 levels: 128 entries never fill)
+constexpr int kTmLanes = 64, kTmVerts = 256, kTmPlanes = 192, kTmList = 128, kTmThreads = 128;
 constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 128, kT1Threads = 128;
 constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 128, kT2Threads = 64;
 using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, false>;
 using Tier0x = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, true>;
+using TierM = Group<kTmLanes, kTmVerts, kTmPlanes, kTmList, true>;
 using Tier1 = Group<kT1Lanes, kT1Verts, kT1Planes, kT1List, true>;
 using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 
@@ -1523,7 +1528,7 @@ using Tier2 = Group<kT2Lanes, kT2Verts, kT2Planes, kT2List, true>;
 // the FP64-capable instance (kExact), whose waves scan the flags in windows
 // of 64 Morton-consecutive cells and take the flagged cells four at a time
 // (neighbours again, so the shared traversal stays coherent).  A cell that
-// outgrows the tier is queued for tier 1.
+// outgrows the tier is queued for tier m.
 // The address of the block's copy of the launch's parameters (in LDS: one
 // copy per block, where a kernel argument whose address is taken would be
 // copied per lane to scratch), opaque to the optimiser: fields read through
@@ -1586,7 +1591,7 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
     const int grp = lane / kT0Lanes;
     if (kExactTier && prm.counters[5] == 0u) return;  // no cell was flagged
     if (kExactTier && prm.counters[5] <= kFewFlagged) {
-        // a few flagged cells: queued for tier 1, with no plane owners (the
+        // a few flagged cells: queued for tier m, with no plane owners (the
         // flags read in a grid-stride loop: no work counter to contend for)
         const int64_t count = prm.end - prm.begin;
         const int64_t step = (int64_t)gridDim.x * kT0Threads;
@@ -1594,7 +1599,7 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
             if (prm.exact_list[c] != 0u) {
                 const uint32_t slot = atomicAdd(&prm.counters[1], 1u);
                 const uint32_t cell = (uint32_t)(prm.begin + c);
-                if (slot < prm.queue_cap) prm.big_list[slot] = cell;
+                if (slot < prm.queue_cap) prm.mid_list[slot] = cell;
                 else truncate_cell(prm, prm.perm[cell]);
                 if (slot < prm.promo_cap)
                     for (int p = 0; p < kPromoPlanes; ++p) prm.promo[(size_t)slot * kPromoPlanes + p] = -1;
@@ -1688,7 +1693,7 @@ __device__ __forceinline__ void tier0_body(const CellParams &prm) {
             } else {  // queue for the large-capacity tier
                 g.count(kStatPromoted);
                 slot = atomicAdd(&P.counters[1], 1u);
-                if (slot < P.queue_cap) P.big_list[slot] = cell;
+                if (slot < P.queue_cap) P.mid_list[slot] = cell;
                 else truncate_cell(P, orig);
             }
         }
@@ -1818,51 +1823,69 @@ __global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellPar
     }
 }
 
-// Tier 1: one cell per wave, the queued cells.
-__global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
-    __shared__ GroupLDS<kT1Verts, kT1Planes> lds[kT1Threads / kWave];
-    __shared__ CandLDS<kT1Lanes> cands[kT1Threads / kWave];
+// Tiers m and 1: one cell per wave, the cells queued in `in` (count in
+// counters[kIn], work counter counters[kIn + 1]); a cell that outgrows the
+// tier goes to `out` (count in counters[kOut]).  kOwners: the queue slots
+// below promo_cap carry the plane owners tier 0 held (offered first).
+template <typename T, int V, int P, int kThreads, int kIn, int kOut, bool kOwners>
+__device__ __forceinline__ void queued_tier_body(const CellParams &prm, const uint32_t *in, uint32_t *out) {
+    __shared__ GroupLDS<V, P> lds[kThreads / kWave];
+    __shared__ CandLDS<kWave> cands[kThreads / kWave];
     __shared__ CellParams sprm;
     if (threadIdx.x == 0) sprm = prm;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    Tier1 g;
+    T g;
     g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
-    const uint32_t queued = min(prm.counters[1], prm.queue_cap);
+    const uint32_t queued = min(prm.counters[kIn], prm.queue_cap);
     while (true) {
         uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&prm.counters[2], 1u);
+        if (lane == 0) k = atomicAdd(&prm.counters[kIn + 1], 1u);
         k = (uint32_t)__shfl((int)k, 0);
         if (k >= queued) break;
         const uint64_t t0 = clock64();
-        const uint32_t cell = prm.big_list[k];
+        const uint32_t cell = in[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
         g.active = true;
         g.start(cell, *reread(&sprm));
-        if (k < prm.promo_cap) g.offer_owners(prm.promo + (size_t)k * kPromoPlanes, kPromoPlanes);
-        g.traverse();
-        const CellParams &P = *reread(&sprm);
-        const uint32_t orig = P.perm[cell];
-        if (lane == 0 && g.S->exact_events) log_exact(P, orig, g.S->exact_events);
-        if (!g.overflow) {
-            const int F = g.stage_row(P);
-            const uint64_t base = pool_take(P, (uint32_t)F);
-            if (base + (uint64_t)F <= P.rows_elems) {
-                if (lane == 0) P.row_offsets[cell] = (int64_t)base;
-                g.write_row(P, orig, P.rows + base, F);
-            } else if (lane == 0) {
-                pool_short(P, cell, orig, base + (uint64_t)F);
-            }
-        } else if (lane == 0) {  // queue for tier 2
-            g.count(kStatPromoted);
-            const uint32_t slot = atomicAdd(&P.counters[3], 1u);
-            if (slot < P.queue_cap) P.huge_list[slot] = cell;
-            else truncate_cell(P, orig);
+        if constexpr (kOwners) {
+            if (k < prm.promo_cap) g.offer_owners(prm.promo + (size_t)k * kPromoPlanes, kPromoPlanes);
         }
-        if (P.cycles && lane == 0)
-            P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        g.traverse();
+        const CellParams &Q = *reread(&sprm);
+        const uint32_t orig = Q.perm[cell];
+        if (lane == 0 && g.S->exact_events) log_exact(Q, orig, g.S->exact_events);
+        if (!g.overflow) {
+            const int F = g.stage_row(Q);
+            const uint64_t base = pool_take(Q, (uint32_t)F);
+            if (base + (uint64_t)F <= Q.rows_elems) {
+                if (lane == 0) Q.row_offsets[cell] = (int64_t)base;
+                g.write_row(Q, orig, Q.rows + base, F);
+            } else if (lane == 0) {
+                pool_short(Q, cell, orig, base + (uint64_t)F);
+            }
+        } else if (lane == 0) {  // queue for the next tier
+            g.count(kStatPromoted);
+            const uint32_t slot = atomicAdd(&Q.counters[kOut], 1u);
+            if (slot < Q.queue_cap) out[slot] = cell;
+            else truncate_cell(Q, orig);
+        }
+        if (Q.cycles && lane == 0)
+            Q.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
     }
     g.flush(prm, 0);
+}
+
+// Tier m: 256 vertices / 192 planes at three waves per SIMD, the cells tier 0
+// (or its twin) gave up on capacity.
+__global__ void __launch_bounds__(kTmThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
+cell_group_mid_kernel(const CellParams prm) {
+    queued_tier_body<TierM, kTmVerts, kTmPlanes, kTmThreads, 1, 8, true>(prm, prm.mid_list, prm.big_list);
+}
+
+// Tier 1: 512 vertices / 255 planes, the cells that outgrow tier m.
+__global__ void __launch_bounds__(kT1Threads) cell_group_big_kernel(const CellParams prm) {
+    queued_tier_body<Tier1, kT1Verts, kT1Planes, kT1Threads, 8, 3, false>(prm, prm.big_list, prm.huge_list);
 }
 
 // Tier 2: one cell per wave, one wave per block (57 KB of LDS), the cells

@@ -177,13 +177,13 @@ def test_library_argument_errors_without_gpu():
     assert lib.lib.pg_compute_cells(10, None, 0, 20, None, None, 0, 16, None, 0, None, None, None, None, None, 0,
                                     1, None, None) == 1
     assert b"out of bounds" in lib.lib.pg_last_error()
-    # the row pool and the cell scratch are sized per cell (12 bytes of
-    # scratch: the FP64 flag and both queues, so no queue can fill; 48 plane
-    # owners for one cell in 32, and 64 more, queued for tier 1; a 20-byte
+    # the row pool and the cell scratch are sized per cell (16 bytes of
+    # scratch: the FP64 flag and three queues, so no queue can fill; 48 plane
+    # owners for one cell in 32, and 64 more, queued for tier m; a 20-byte
     # record for one removal test per two cells, and 1024 more, that tier 0
     # took by its FP32 sign)
     assert lib.lib.pg_row_pool_elems(1000, ctypes.byref(out)) == 0 and out.value == 32 * 1000 + 65536
-    assert lib.lib.pg_cell_scratch_bytes(1000, ctypes.byref(out)) == 0 and out.value == 256 + 12 * 1000 + 4 * 48 * (1000 // 32 + 64) + 20 * (1000 // 2 + 1024)
+    assert lib.lib.pg_cell_scratch_bytes(1000, ctypes.byref(out)) == 0 and out.value == 256 + 16 * 1000 + 4 * 48 * (1000 // 32 + 64) + 20 * (1000 // 2 + 1024)
     # the index layout (host only): increasing offsets inside the workspace
     offs = (ctypes.c_uint64 * 6)()
     assert lib.lib.pg_index_layout(1000, offs) == 0
