@@ -5,7 +5,7 @@
 // Four instances (capacity tiers):
 //   tier 0: G = 16 lanes, 64 vertex slots, 48 plane slots (about 3.3 KB of
 //           LDS per cell, 4 cells per wave) -- every cell starts here;
-//   tier m: G = 64 lanes, 256 vertex slots, 192 plane slots (one cell per
+//   tier m: G = 64 lanes, 192 vertex slots, 128 plane slots (one cell per
 //           wave, 12 waves per CU) -- the few cells that outgrow tier 0
 //           (cells with more than about 34 faces);
 //   tier 1: G = 64 lanes, 512 vertex slots, 255 plane slots (one cell per
@@ -1513,7 +1513,7 @@ __device__ __forceinline__ int batch_find(const int64_t *off, int n_clouds, int6
 constexpr int kT0Lanes = 16, kT0Verts = 64, kT0Planes = 48, kT0List = 128, kT0Threads = 256;
 // (a depth-first stack holds at most one entry per level of the hierarchy,
 // fewer than 64 levels: 128 entries never fill)
-constexpr int kTmLanes = 64, kTmVerts = 256, kTmPlanes = 192, kTmList = 128, kTmThreads = 128;
+constexpr int kTmLanes = 64, kTmVerts = 192, kTmPlanes = 128, kTmList = 128, kTmThreads = 128;
 constexpr int kT1Lanes = 64, kT1Verts = 512, kT1Planes = 255, kT1List = 128, kT1Threads = 128;
 constexpr int kT2Lanes = 64, kT2Verts = 1024, kT2Planes = 1023, kT2List = 128, kT2Threads = 64;
 using Tier0 = Group<kT0Lanes, kT0Verts, kT0Planes, kT0List, false>;
@@ -1876,7 +1876,7 @@ __device__ __forceinline__ void queued_tier_body(const CellParams &prm, const ui
     g.flush(prm, 0);
 }
 
-// Tier m: 256 vertices / 192 planes at three waves per SIMD, the cells tier 0
+// Tier m: 192 vertices / 128 planes at three waves per SIMD, the cells tier 0
 // (or its twin) gave up on capacity.
 __global__ void __launch_bounds__(kTmThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
 cell_group_mid_kernel(const CellParams prm) {
