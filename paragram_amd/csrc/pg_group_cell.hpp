@@ -1826,18 +1826,20 @@ __global__ void __launch_bounds__(256) cell_tentative_check_kernel(const CellPar
 // Tiers m and 1: one cell per wave, the cells queued in `in` (count in
 // counters[kIn], work counter counters[kIn + 1]); a cell that outgrows the
 // tier goes to `out` (count in counters[kOut]).  kOwners: the queue slots
-// below promo_cap carry the plane owners tier 0 held (offered first).
+// below promo_cap carry the plane owners tier 0 held (offered first).  An
+// empty queue ends the launch before any set-up.
 template <typename T, int V, int P, int kThreads, int kIn, int kOut, bool kOwners>
 __device__ __forceinline__ void queued_tier_body(const CellParams &prm, const uint32_t *in, uint32_t *out) {
     __shared__ GroupLDS<V, P> lds[kThreads / kWave];
     __shared__ CandLDS<kWave> cands[kThreads / kWave];
     __shared__ CellParams sprm;
+    const uint32_t queued = min(prm.counters[kIn], prm.queue_cap);
+    if (queued == 0u) return;
     if (threadIdx.x == 0) sprm = prm;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     T g;
     g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
-    const uint32_t queued = min(prm.counters[kIn], prm.queue_cap);
     while (true) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&prm.counters[kIn + 1], 1u);
@@ -1894,12 +1896,13 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
     __shared__ GroupLDS<kT2Verts, kT2Planes> lds;
     __shared__ CandLDS<kT2Lanes> cands;
     __shared__ CellParams sprm;
+    const uint32_t queued = min(prm.counters[3], prm.queue_cap);
+    if (queued == 0u) return;
     if (threadIdx.x == 0) sprm = prm;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     Tier2 g;
     g.setup(prm, &lds, &cands);
-    const uint32_t queued = min(prm.counters[3], prm.queue_cap);
     while (true) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
