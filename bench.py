@@ -218,7 +218,7 @@ def measure_extra(lib, dev, cpu_check: bool = True) -> dict:
         return _engine.split_batch(*out, clouds)
 
     ms, all_ms, parts = _time_calls(batch_call, 1, 3)
-    # the north star's chosen form: the same batch started from the learned radius
+    # the learned + batched form: the same batch started from the learned radius
     instr_l = _engine.Instrument()
 
     def batch_learned():
