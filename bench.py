@@ -52,7 +52,7 @@ WORKLOADS = {
 }
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "cell_kernel_traffic.json")
 CELL_KERNELS = ("cell_group_kernel", "cell_tentative_check_kernel", "cell_group_exact_kernel",
-                "cell_group_exact_batch_kernel", "cell_group_big_kernel",
+                "cell_group_exact_batch_kernel", "cell_group_mid_kernel", "cell_group_big_kernel",
                 "cell_group_huge_kernel")  # (the FP64 twin's batch form launches for batches)
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 

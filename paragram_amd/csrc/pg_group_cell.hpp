@@ -1838,13 +1838,15 @@ __device__ __forceinline__ void queued_tier_body(const CellParams &prm, const ui
     if (threadIdx.x == 0) sprm = prm;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
+    // the wave's first cell before its set-up: a wave without one ends here
+    // (after the block's only barrier), before it writes any spilled register
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&prm.counters[kIn + 1], 1u);
+    k = (uint32_t)__shfl((int)k, 0);
+    if (k >= queued) return;
     T g;
     g.setup(prm, &lds[threadIdx.x / kWave], &cands[threadIdx.x / kWave]);
     while (true) {
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&prm.counters[kIn + 1], 1u);
-        k = (uint32_t)__shfl((int)k, 0);
-        if (k >= queued) break;
         const uint64_t t0 = clock64();
         const uint32_t cell = in[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
@@ -1874,6 +1876,9 @@ __device__ __forceinline__ void queued_tier_body(const CellParams &prm, const ui
         }
         if (Q.cycles && lane == 0)
             Q.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (lane == 0) k = atomicAdd(&prm.counters[kIn + 1], 1u);
+        k = (uint32_t)__shfl((int)k, 0);
+        if (k >= queued) break;
     }
     g.flush(prm, 0);
 }
@@ -1901,13 +1906,13 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
     if (threadIdx.x == 0) sprm = prm;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
+    uint32_t k = 0;  // the wave's first cell before its set-up, as in the tiers above
+    if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
+    k = (uint32_t)__shfl((int)k, 0);
+    if (k >= queued) return;
     Tier2 g;
     g.setup(prm, &lds, &cands);
     while (true) {
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
-        k = (uint32_t)__shfl((int)k, 0);
-        if (k >= queued) break;
         const uint64_t t0 = clock64();
         const uint32_t cell = prm.huge_list[k];
         if (prm.clouds != nullptr) g.set_cloud(&prm.clouds[uniform(batch_find(prm.cloud_cells, prm.n_clouds, cell))]);
@@ -1931,6 +1936,9 @@ __global__ void __launch_bounds__(kT2Threads) cell_group_huge_kernel(const CellP
         }
         if (P.cycles && lane == 0)
             P.cycles[orig] = 0x80000000u | (uint32_t)min(clock64() - t0, (uint64_t)0x7fffffff);
+        if (lane == 0) k = atomicAdd(&prm.counters[4], 1u);
+        k = (uint32_t)__shfl((int)k, 0);
+        if (k >= queued) break;
     }
     g.flush(prm, 0);
 }
